@@ -1,0 +1,260 @@
+"""Initial conditions and phi producers for the benchmark workloads.
+
+These are the "reference path" inputs: they run once on the host and are not
+timed.  Host arrays use the reference layout -- a state of the reference's
+shape ``(5, Nx, Ny, Nz)`` column-major is held as a C-contiguous numpy array of
+shape ``(Nz, Ny, Nx, 5)`` (identical bytes); a potential of the reference's
+shape ``(nx, ny, nz)`` is held as ``(nz, ny, nx)``.
+
+Restated from:
+  sedov_ic_3d!            src/euler3d.jl:275-316
+  Sod set-up              test/test_sod3d.jl:79-104
+  lane_emden              src/stellar_ic.jl:34-63
+  polytrope_ic_3d!        src/stellar_ic.jl:87-178
+  bh_potential            src/gravity_bh.jl:28-31
+  solve_poisson_isolated  src/self_gravity.jl:35-76   (numpy FFT, not FFTW)
+  _prolong_scalar_trilinear!  src/fmr_gravity.jl:70-106
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+NG = 3
+
+
+def empty_state(nx: int, ny: int, nz: int) -> np.ndarray:
+    return np.zeros((nz + 2 * NG, ny + 2 * NG, nx + 2 * NG, 5), dtype=np.float64)
+
+
+def active(U: np.ndarray) -> np.ndarray:
+    """View of the active cells, shape (nz, ny, nx, 5)."""
+    return U[NG:-NG, NG:-NG, NG:-NG, :]
+
+
+def _centres(n: int, d: float, offset: float) -> np.ndarray:
+    # (i - ng - 0.5) * d + offset for the 1-based padded index i of active cells
+    return (np.arange(1, n + 1, dtype=np.float64) - 0.5) * d + offset
+
+
+# ---------------------------------------------------------------------------
+# Sod tube (test/test_sod3d.jl:79-104), generalised to any left/right states.
+
+def sod_3d(direction: str = "x", n_normal: int = 128, n_trans: int = 4, gamma: float = 1.4,
+           left=(1.0, 0.0, 1.0), right=(0.125, 0.0, 0.1), length: float = 1.0):
+    """Returns (U, (nx, ny, nz), (dx, dy, dz))."""
+    axis = "xyz".index(direction)
+    n = [n_trans, n_trans, n_trans]
+    n[axis] = n_normal
+    nx, ny, nz = n
+    d = [length / nx, length / ny, length / nz]
+    U = empty_state(nx, ny, nz)
+    coord = (np.arange(1, n_normal + 1, dtype=np.float64) - 0.5) * d[axis] - 0.5 * length
+    prof = np.empty((n_normal, 5))
+    for a, c in enumerate(coord):
+        r, v, p = left if c < 0.0 else right
+        prof[a] = (r, 0.0, 0.0, 0.0, p / (gamma - 1) + (0.5 * r) * (v * v))
+        prof[a, 1 + axis] = r * v
+    A = active(U)
+    shape = [1, 1, 1]
+    shape[2 - axis] = n_normal
+    A[...] = prof.reshape(shape + [5])
+    return U, (nx, ny, nz), tuple(d)
+
+
+# ---------------------------------------------------------------------------
+# Sedov (src/euler3d.jl:275-316).  Note that the reference deposits
+# E_blast*(gamma-1) of energy; pass E_blast = E0/(gamma-1) to deposit E0.
+
+def sedov_ic_3d(nx, ny, nz, dx, dy, dz, gamma, E_blast=1.0, r_inject=None, rho_bg=1.0,
+                P_floor=1e-5, x_offset=0.0, y_offset=0.0, z_offset=0.0):
+    U = empty_state(nx, ny, nz)
+    r_inj = 2.5 * min(dx, dy, dz) if r_inject is None else float(r_inject)
+    A = active(U)
+    E_bg = P_floor / (gamma - 1)
+    A[..., 0] = rho_bg
+    A[..., 4] = E_bg
+    xc = _centres(nx, dx, x_offset)[None, None, :]
+    yc = _centres(ny, dy, y_offset)[None, :, None]
+    zc = _centres(nz, dz, z_offset)[:, None, None]
+    r = np.sqrt(xc * xc + yc * yc + zc * zc)
+    inside = r <= r_inj
+    V_inj = 0.0
+    dV = dx * dy * dz
+    for _ in range(int(inside.sum())):      # sequential accumulation, as the reference
+        V_inj += dV
+    dE = E_blast * (gamma - 1) / V_inj if V_inj > 0.0 else 0.0
+    A[..., 4] = np.where(inside, E_bg + dE, E_bg)
+    return U
+
+
+# ---------------------------------------------------------------------------
+# Turbulent periodic box: rho = exp(s), s a Gaussian random field with
+# P(k) ~ k^(-11/3) for 2 <= |k| <= 32, sigma(s) = 0.5; velocity components are
+# independent fields with the same spectrum scaled to rms Mach 2; p = 1.
+
+def gaussian_field(n: int, seed: int, kmin: float = 2.0, kmax: float = 32.0,
+                   slope: float = -11.0 / 3.0) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kz = np.fft.rfftfreq(n, d=1.0 / n)
+    kk = np.sqrt(k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    amp = np.zeros_like(kk)
+    band = (kk >= kmin) & (kk <= kmax)
+    amp[band] = kk[band] ** (slope / 2.0)
+    spec = (rng.standard_normal(kk.shape) + 1j * rng.standard_normal(kk.shape)) * amp
+    f = np.fft.irfftn(spec, s=(n, n, n))
+    f -= f.mean()
+    f /= f.std()
+    return f
+
+
+def turbulent_box(n: int = 256, gamma: float = 5.0 / 3.0, mach: float = 2.0, sigma_s: float = 0.5,
+                  seed: int = 1234, p0: float = 1.0):
+    """Returns (U, (n,n,n), (dx,dx,dx)) on the unit periodic cube."""
+    s = sigma_s * gaussian_field(n, seed)
+    rho = np.exp(s)
+    vel = [gaussian_field(n, seed + 1 + a) for a in range(3)]
+    # scale so that sqrt(<|v|^2 rho / (gamma p)>) = mach
+    v2 = vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2
+    scale = mach / math.sqrt(float(np.mean(v2 * rho / (gamma * p0))))
+    U = empty_state(n, n, n)
+    A = active(U)
+    A[..., 0] = rho
+    KE = np.zeros_like(rho)
+    for a in range(3):
+        v = vel[a] * scale
+        A[..., 1 + a] = rho * v
+        KE += v * v
+    A[..., 4] = p0 / (gamma - 1) + 0.5 * rho * KE
+    dx = 1.0 / n
+    return U, (n, n, n), (dx, dx, dx)
+
+
+# ---------------------------------------------------------------------------
+# Lane-Emden polytrope (src/stellar_ic.jl:34-178)
+
+def lane_emden(n: float, dxi: float = 1e-3):
+    nf = float(n)
+    xi0 = dxi
+    th = 1.0 - xi0 ** 2 / 6.0 + nf * xi0 ** 4 / 120.0
+    dth = -xi0 / 3.0 + nf * xi0 ** 3 / 30.0
+    xis, ths, dths = [xi0], [th], [dth]
+    xi = xi0
+
+    def f(x, t, d):
+        return -(max(t, 0.0)) ** nf - 2.0 * d / x
+
+    while th > 0.0:
+        k1t = dxi * dth
+        k1d = dxi * f(xi, th, dth)
+        k2t = dxi * (dth + k1d / 2)
+        k2d = dxi * f(xi + dxi / 2, th + k1t / 2, dth + k1d / 2)
+        k3t = dxi * (dth + k2d / 2)
+        k3d = dxi * f(xi + dxi / 2, th + k2t / 2, dth + k2d / 2)
+        k4t = dxi * (dth + k3d)
+        k4d = dxi * f(xi + dxi, th + k3t, dth + k3d)
+        th += (k1t + 2 * k2t + 2 * k3t + k4t) / 6
+        dth += (k1d + 2 * k2d + 2 * k3d + k4d) / 6
+        xi += dxi
+        xis.append(xi)
+        ths.append(th)
+        dths.append(dth)
+    return np.array(xis), np.array(ths), np.array(dths)
+
+
+def polytrope_scales(gamma: float, R_star: float, rho_c: float):
+    """(M_star, r_scale, K, xi_1) of the polytrope with central density rho_c."""
+    n = 1.0 / (gamma - 1.0)
+    xis, ths, dths = lane_emden(n)
+    xi1, dth1 = xis[-1], dths[-1]
+    omega = -xi1 ** 2 * dth1
+    r_scale = R_star / xi1
+    M_star = rho_c * 4 * math.pi * r_scale ** 3 * omega
+    K = 4 * math.pi * r_scale ** 2 * rho_c ** (1.0 - 1.0 / n) / (n + 1.0)
+    return M_star, r_scale, K, xi1
+
+
+def polytrope_ic_3d(nx, ny, nz, dx, dy, dz, gamma, R_star=1.0, rho_c=1.0, x0=0.0, y0=0.0,
+                    z0=0.0, rho_floor=1e-10, P_floor=1e-12):
+    n = 1.0 / (gamma - 1.0)
+    xis, ths, _ = lane_emden(n)
+    xi1 = xis[-1]
+    M_star, r_scale, K, _ = polytrope_scales(gamma, R_star, rho_c)
+    dxi = xis[1] - xis[0]
+    xc = _centres(nx, dx, x0)[None, None, :]
+    yc = _centres(ny, dy, y0)[None, :, None]
+    zc = _centres(nz, dz, z0)[:, None, None]
+    r = np.sqrt(xc * xc + yc * yc + zc * zc)
+    xq = r / r_scale
+    i_f = xq / dxi
+    i = np.clip(np.floor(i_f).astype(np.int64), 1, len(xis) - 1)
+    a = i_f - i
+    th = (1.0 - a) * ths[i - 1] + a * ths[np.minimum(i, len(ths) - 1)]
+    th = np.where(xq <= 0.0, 1.0, np.where(xq >= xi1, 0.0, th))
+    inside = th > 0.0
+    rho = np.where(inside, rho_c * np.maximum(th, 0.0) ** n, rho_floor)
+    P = np.where(inside, K * rho ** (1.0 + 1.0 / n), P_floor)
+    U = empty_state(nx, ny, nz)
+    A = active(U)
+    A[..., 0] = rho
+    A[..., 4] = P / (gamma - 1.0)
+    return U
+
+
+# ---------------------------------------------------------------------------
+# Potentials
+
+def point_mass_phi(nx, ny, nz, dx, dy, dz, x0, y0, z0, masses):
+    """Sum of Plummer potentials -M/sqrt(r^2 + eps^2) (src/gravity_bh.jl:28-31)
+    sampled at cell centres; masses = [(M, (px,py,pz), eps), ...]."""
+    xc = _centres(nx, dx, x0)[None, None, :]
+    yc = _centres(ny, dy, y0)[None, :, None]
+    zc = _centres(nz, dz, z0)[:, None, None]
+    phi = np.zeros((nz, ny, nx))
+    for M, (px, py, pz), eps in masses:
+        ddx, ddy, ddz = xc - px, yc - py, zc - pz
+        phi -= M / np.sqrt(ddx ** 2 + ddy ** 2 + ddz ** 2 + eps ** 2)
+    return phi
+
+
+def solve_poisson_isolated(rho: np.ndarray, dx: float, dy: float, dz: float) -> np.ndarray:
+    """Hockney-Eastwood zero-padded convolution with K = -1/sqrt(r^2+eps^2),
+    eps^2 = (dx dy dz)^(2/3) (src/self_gravity.jl:35-76).  rho is (nz,ny,nx)."""
+    nz, ny, nx = rho.shape
+    Nx, Ny, Nz = 2 * nx, 2 * ny, 2 * nz
+    dV = dx * dy * dz
+    eps2 = (dx * dy * dz) ** (2.0 / 3.0)
+    fi = np.where(np.arange(Nx) < nx, np.arange(Nx), np.arange(Nx) - Nx).astype(np.float64)
+    fj = np.where(np.arange(Ny) < ny, np.arange(Ny), np.arange(Ny) - Ny).astype(np.float64)
+    fk = np.where(np.arange(Nz) < nz, np.arange(Nz), np.arange(Nz) - Nz).astype(np.float64)
+    r2 = ((fk * dz) ** 2)[:, None, None] + ((fj * dy) ** 2)[None, :, None] + \
+         ((fi * dx) ** 2)[None, None, :]
+    K = -1.0 / np.sqrt(r2 + eps2)
+    pad = np.zeros((Nz, Ny, Nx))
+    pad[:nz, :ny, :nx] = rho
+    phi = np.fft.irfftn(np.fft.rfftn(K) * np.fft.rfftn(pad), s=(Nz, Ny, Nx)) * dV
+    return np.ascontiguousarray(phi[:nz, :ny, :nx])
+
+
+def prolong_scalar_trilinear(phi_c: np.ndarray, lo: tuple, ratio: int, nf: tuple) -> np.ndarray:
+    """Trilinear prolongation of an active-cell scalar onto a nested box
+    (src/fmr_gravity.jl:70-106).  lo = (ci_lo, cj_lo, ck_lo), 1-based;
+    nf = (nxf, nyf, nzf).  Arrays are (z, y, x)."""
+    ncz, ncy, ncx = phi_c.shape
+    out = []
+    for axis, (c_lo, n_f, n_c) in enumerate(zip(lo, nf, (ncx, ncy, ncz))):
+        pos = (c_lo - 1) + (np.arange(1, n_f + 1) - 0.5) / ratio + 0.5
+        ic = np.clip(np.floor(pos).astype(np.int64), 1, n_c - 1)
+        out.append((ic, pos - ic))
+    (ix, fx), (iy, fy), (iz, fz) = out
+    fx = fx[None, None, :]
+    fy = fy[None, :, None]
+    fz = fz[:, None, None]
+    P = lambda a, b, c: phi_c[(iz + c - 1)[:, None, None], (iy + b - 1)[None, :, None],
+                              (ix + a - 1)[None, None, :]]
+    return ((1 - fx) * (1 - fy) * (1 - fz) * P(0, 0, 0) + fx * (1 - fy) * (1 - fz) * P(1, 0, 0)
+            + (1 - fx) * fy * (1 - fz) * P(0, 1, 0) + fx * fy * (1 - fz) * P(1, 1, 0)
+            + (1 - fx) * (1 - fy) * fz * P(0, 0, 1) + fx * (1 - fy) * fz * P(1, 0, 1)
+            + (1 - fx) * fy * fz * P(0, 1, 1) + fx * fy * fz * P(1, 1, 1))

@@ -1,0 +1,1066 @@
+/*
+ * bsn_oracle.c -- CPU FP64 restatement of the Binary-Supernova Euler hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle and the CPU
+ * baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker.  The product (the HIP library in
+ * binary-supernova_amd/csrc) never links or calls it.
+ *
+ * Every function restates one reference function, in the reference's own
+ * operation order, so that a build with -ffp-contract=off reproduces the
+ * reference's IEEE-754 FP64 results bit for bit:
+ *   src/weno5.jl:14-61        wenoz_weights, weno5_left, weno5_right
+ *   src/hllc.jl:17-208        _hllc_normal, hllc_flux_x/y/z
+ *   src/euler3d.jl:33-260     ghost fills, floors, _cell_pressure, _EL_from_P,
+ *                             flux divergence, cfl_dt_3d, euler3d_rhs!, euler3d_step!
+ *   src/gpu_kernels.jl:26-191 the per-cell kernel bodies
+ *   src/gpu_kernels.jl:291-319 _sg_gradient_kernel!   (phi source)
+ *   src/rk3.jl:14-31          the Shu-Osher stage combinations
+ *   src/fmr3d.jl:32-622       two-level FMR (prolong, restrict, flux registers,
+ *                             Berger-Colella correction, sub-cycled step, CFL)
+ *   src/diagnostics.jl:21-58  conservation totals (summation order not pinned)
+ *
+ * Layout is the reference's: U is (5, Nx, Ny, Nz), column-major with the field
+ * index q fastest, N? = n? + 2*NG, NG = 3.  Indices below are 0-based: the
+ * reference's 1-based index a maps to a-1.
+ *
+ * Julia semantics that matter for bit parity:
+ *   max/min propagate NaN and order -0.0 < +0.0 (jl_max / jl_min);
+ *   clamp(x, lo, hi) = x > hi ? hi : (x < lo ? lo : x);
+ *   x^2 is x*x; a*b*c is (a*b)*c; a+b+c is (a+b)+c; no FMA contraction.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "bsn_oracle.h"
+
+#define NG 3
+
+/* ------------------------------------------------------------------------ */
+/* Julia scalar semantics                                                    */
+
+static inline double jl_max(double x, double y)
+{
+    double diff = x - y;
+    if (isnan(x) || isnan(y)) return diff;
+    return signbit(diff) ? y : x;
+}
+
+static inline double jl_min(double x, double y)
+{
+    double diff = x - y;
+    if (isnan(x) || isnan(y)) return diff;
+    return signbit(diff) ? x : y;
+}
+
+static inline double jl_clamp(double x, double lo, double hi)
+{
+    return x > hi ? hi : (x < lo ? lo : x);
+}
+
+static inline int64_t U_IDX(int q, int64_t i, int64_t j, int64_t k,
+                            int64_t Nx, int64_t Ny)
+{
+    return (int64_t)q + 5 * (i + Nx * (j + Ny * k));
+}
+
+/* ------------------------------------------------------------------------ */
+/* WENO5-Z (src/weno5.jl:14-61)                                              */
+
+void orc_wenoz_weights(double b0, double b1, double b2,
+                       double d0, double d1, double d2,
+                       double *w0, double *w1, double *w2)
+{
+    const double eps = 1e-36;                 /* weno5.jl:14 */
+    double tau5 = fabs(b0 - b2);              /* weno5.jl:15 */
+    double t0 = tau5 / (eps + b0);
+    double t1 = tau5 / (eps + b1);
+    double t2 = tau5 / (eps + b2);
+    double a0 = d0 * (1.0 + t0 * t0);         /* weno5.jl:16-18 */
+    double a1 = d1 * (1.0 + t1 * t1);
+    double a2 = d2 * (1.0 + t2 * t2);
+    double asum = a0 + a1 + a2;               /* weno5.jl:19 */
+    *w0 = a0 / asum;
+    *w1 = a1 / asum;
+    *w2 = a2 / asum;
+}
+
+double orc_weno5_left(double vm2, double vm1, double v0, double vp1, double vp2)
+{
+    /* weno5.jl:25-41 */
+    double q0 = (1.0 / 3.0) * vm2 + (-7.0 / 6.0) * vm1 + (11.0 / 6.0) * v0;
+    double q1 = (-1.0 / 6.0) * vm1 + (5.0 / 6.0) * v0 + (1.0 / 3.0) * vp1;
+    double q2 = (1.0 / 3.0) * v0 + (5.0 / 6.0) * vp1 + (-1.0 / 6.0) * vp2;
+    double a, b;
+    a = vm2 - 2.0 * vm1 + v0;   b = vm2 - 4.0 * vm1 + 3.0 * v0;
+    double beta0 = (13.0 / 12.0) * (a * a) + (1.0 / 4.0) * (b * b);
+    a = vm1 - 2.0 * v0 + vp1;   b = vm1 - vp1;
+    double beta1 = (13.0 / 12.0) * (a * a) + (1.0 / 4.0) * (b * b);
+    a = v0 - 2.0 * vp1 + vp2;   b = 3.0 * v0 - 4.0 * vp1 + vp2;
+    double beta2 = (13.0 / 12.0) * (a * a) + (1.0 / 4.0) * (b * b);
+    double w0, w1, w2;
+    orc_wenoz_weights(beta0, beta1, beta2, 1.0 / 10.0, 6.0 / 10.0, 3.0 / 10.0,
+                      &w0, &w1, &w2);
+    return w0 * q0 + w1 * q1 + w2 * q2;
+}
+
+double orc_weno5_right(double vm1, double v0, double vp1, double vp2, double vp3)
+{
+    /* weno5.jl:45-61 */
+    double q0 = (-1.0 / 6.0) * vm1 + (5.0 / 6.0) * v0 + (1.0 / 3.0) * vp1;
+    double q1 = (1.0 / 3.0) * v0 + (5.0 / 6.0) * vp1 + (-1.0 / 6.0) * vp2;
+    double q2 = (11.0 / 6.0) * vp1 + (-7.0 / 6.0) * vp2 + (1.0 / 3.0) * vp3;
+    double a, b;
+    a = vm1 - 2.0 * v0 + vp1;   b = vm1 - vp1;
+    double beta0 = (13.0 / 12.0) * (a * a) + (1.0 / 4.0) * (b * b);
+    a = v0 - 2.0 * vp1 + vp2;   b = v0 - 4.0 * vp1 + 3.0 * vp2;
+    double beta1 = (13.0 / 12.0) * (a * a) + (1.0 / 4.0) * (b * b);
+    a = vp1 - 2.0 * vp2 + vp3;  b = 3.0 * vp1 - 4.0 * vp2 + vp3;
+    double beta2 = (13.0 / 12.0) * (a * a) + (1.0 / 4.0) * (b * b);
+    double w0, w1, w2;
+    orc_wenoz_weights(beta0, beta1, beta2, 3.0 / 10.0, 6.0 / 10.0, 1.0 / 10.0,
+                      &w0, &w1, &w2);
+    return w0 * q0 + w1 * q1 + w2 * q2;
+}
+
+/* ------------------------------------------------------------------------ */
+/* HLLC (src/hllc.jl:17-165)                                                 */
+
+void orc_hllc_normal(double rL, double mnL, double mt1L, double mt2L, double EL,
+                     double rR, double mnR, double mt1R, double mt2R, double ER,
+                     double g, double F[5])
+{
+    /* hllc.jl:23-32 -- state reset when rho <= 0 */
+    if (rL <= 0.0) { rL = 1e-30; mnL = 0.0; mt1L = 0.0; mt2L = 0.0; EL = 0.0; }
+    else            { rL = jl_max(rL, 1e-30); }
+    if (rR <= 0.0) { rR = 1e-30; mnR = 0.0; mt1R = 0.0; mt2R = 0.0; ER = 0.0; }
+    else            { rR = jl_max(rR, 1e-30); }
+
+    /* hllc.jl:40-50 -- |v| <= 100 cap */
+    double m2L = mnL * mnL + mt1L * mt1L + mt2L * mt2L;
+    if (m2L > (rL * rL) * 1e4) {
+        double s = (rL * 100.0) / sqrt(m2L);
+        mnL *= s; mt1L *= s; mt2L *= s;
+    }
+    double m2R = mnR * mnR + mt1R * mt1R + mt2R * mt2R;
+    if (m2R > (rR * rR) * 1e4) {
+        double s = (rR * 100.0) / sqrt(m2R);
+        mnR *= s; mt1R *= s; mt2R *= s;
+    }
+
+    /* hllc.jl:58-61 */
+    double KE_L = jl_min((0.5 * (mnL * mnL + mt1L * mt1L + mt2L * mt2L)) / rL,
+                         (0.5 * rL) * 1e4);
+    double KE_R = jl_min((0.5 * (mnR * mnR + mt1R * mt1R + mt2R * mt2R)) / rR,
+                         (0.5 * rR) * 1e4);
+    EL = jl_max(EL, KE_L);
+    ER = jl_max(ER, KE_R);
+
+    /* hllc.jl:63-71 */
+    double vnL = mnL / rL, vt1L = mt1L / rL, vt2L = mt2L / rL;
+    double vnR = mnR / rR, vt1R = mt1R / rR, vt2R = mt2R / rR;
+    double pL = (g - 1.0) * (EL - KE_L);
+    double pR = (g - 1.0) * (ER - KE_R);
+    double csL = sqrt((g * pL) / rL);
+    double csR = sqrt((g * pR) / rR);
+
+    /* hllc.jl:77-88 -- PVRS / Davis wave speeds */
+    double SL, SR;
+    if (pL > 0.0 && pR > 0.0) {
+        double r_avg = 0.5 * (rL + rR);
+        double cs_avg = 0.5 * (csL + csR);
+        double p_star = jl_max(0.0, 0.5 * (pL + pR) - ((0.5 * (vnR - vnL)) * r_avg) * cs_avg);
+        double gfac = (g + 1.0) / (2.0 * g);
+        double qL = p_star > pL ? sqrt(1.0 + gfac * (p_star / pL - 1.0)) : 1.0;
+        double qR = p_star > pR ? sqrt(1.0 + gfac * (p_star / pR - 1.0)) : 1.0;
+        SL = vnL - csL * qL;
+        SR = vnR + csR * qR;
+    } else {
+        SL = jl_min(vnL - csL, vnR - csR);
+        SR = jl_max(vnL + csL, vnR + csR);
+    }
+
+    /* hllc.jl:91-94 */
+    double denom = rL * (SL - vnL) - rR * (SR - vnR);
+    if (fabs(denom) < 1e-100) { F[0] = F[1] = F[2] = F[3] = F[4] = 0.0; return; }
+    double S_star = ((pR - pL) + (rL * vnL) * (SL - vnL) - (rR * vnR) * (SR - vnR)) / denom;
+
+    /* hllc.jl:97-105 */
+    double FrL = mnL;
+    double FmnL = rL * (vnL * vnL) + pL;
+    double Fmt1L = mt1L * vnL;
+    double Fmt2L = mt2L * vnL;
+    double FEL = (EL + pL) * vnL;
+    if (SL >= 0.0) { F[0] = FrL; F[1] = FmnL; F[2] = Fmt1L; F[3] = Fmt2L; F[4] = FEL; return; }
+
+    /* hllc.jl:107-115 */
+    double FrR = mnR;
+    double FmnR = rR * (vnR * vnR) + pR;
+    double Fmt1R = mt1R * vnR;
+    double Fmt2R = mt2R * vnR;
+    double FER = (ER + pR) * vnR;
+    if (SR <= 0.0) { F[0] = FrR; F[1] = FmnR; F[2] = Fmt1R; F[3] = Fmt2R; F[4] = FER; return; }
+
+    /* hllc.jl:121-128 -- HLL fallback for density ratio > 3 */
+    if (jl_max(rL, rR) > 3.0 * jl_min(rL, rR)) {
+        double inv_dS = 1.0 / (SR - SL);
+        double SLSR = SL * SR;
+        F[0] = (SR * FrL - SL * FrR + SLSR * (rR - rL)) * inv_dS;
+        F[1] = (SR * FmnL - SL * FmnR + SLSR * (mnR - mnL)) * inv_dS;
+        F[2] = (SR * Fmt1L - SL * Fmt1R + SLSR * (mt1R - mt1L)) * inv_dS;
+        F[3] = (SR * Fmt2L - SL * Fmt2R + SLSR * (mt2R - mt2L)) * inv_dS;
+        F[4] = (SR * FEL - SL * FER + SLSR * (ER - EL)) * inv_dS;
+        return;
+    }
+
+    /* hllc.jl:131-164 -- HLLC star states */
+    if (S_star >= 0.0) {
+        double denom_L = rL * (SL - vnL);
+        double chi = denom_L / (SL - S_star);
+        double rsL = chi, mnsL = chi * S_star, mt1sL = chi * vt1L, mt2sL = chi * vt2L;
+        double EsL = fabs(denom_L) > 1e-100
+                   ? chi * (EL / rL + (S_star - vnL) * (S_star + pL / denom_L)) : 0.0;
+        F[0] = FrL + SL * (rsL - rL);
+        F[1] = FmnL + SL * (mnsL - mnL);
+        F[2] = Fmt1L + SL * (mt1sL - mt1L);
+        F[3] = Fmt2L + SL * (mt2sL - mt2L);
+        F[4] = FEL + SL * (EsL - EL);
+    } else {
+        double denom_R = rR * (SR - vnR);
+        double chi = denom_R / (SR - S_star);
+        double rsR = chi, mnsR = chi * S_star, mt1sR = chi * vt1R, mt2sR = chi * vt2R;
+        double EsR = fabs(denom_R) > 1e-100
+                   ? chi * (ER / rR + (S_star - vnR) * (S_star + pR / denom_R)) : 0.0;
+        F[0] = FrR + SR * (rsR - rR);
+        F[1] = FmnR + SR * (mnsR - mnR);
+        F[2] = Fmt1R + SR * (mt1sR - mt1R);
+        F[3] = Fmt2R + SR * (mt2sR - mt2R);
+        F[4] = FER + SR * (EsR - ER);
+    }
+}
+
+/* hllc.jl:176-208 -- rotations; dir 0 = x, 1 = y, 2 = z */
+void orc_hllc_flux(int dir, const double L[5], const double R[5], double g, double F[5])
+{
+    double G[5];
+    if (dir == 0) {
+        orc_hllc_normal(L[0], L[1], L[2], L[3], L[4], R[0], R[1], R[2], R[3], R[4], g, G);
+        F[0] = G[0]; F[1] = G[1]; F[2] = G[2]; F[3] = G[3]; F[4] = G[4];
+    } else if (dir == 1) {   /* normal = y, t1 = x, t2 = z */
+        orc_hllc_normal(L[0], L[2], L[1], L[3], L[4], R[0], R[2], R[1], R[3], R[4], g, G);
+        F[0] = G[0]; F[1] = G[2]; F[2] = G[1]; F[3] = G[3]; F[4] = G[4];
+    } else {                 /* normal = z, t1 = x, t2 = y */
+        orc_hllc_normal(L[0], L[3], L[1], L[2], L[4], R[0], R[3], R[1], R[2], R[4], g, G);
+        F[0] = G[0]; F[1] = G[2]; F[2] = G[3]; F[3] = G[1]; F[4] = G[4];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pressure helpers (src/euler3d.jl:86-106)                                  */
+
+double orc_cell_pressure(double r, double mx, double my, double mz, double E, double g)
+{
+    r = jl_max(r, 1e-30);
+    double KE = (0.5 * (mx * mx + my * my + mz * mz)) / r;
+    return jl_max((g - 1.0) * (E - KE), 0.0);
+}
+
+double orc_EL_from_P(double PL, double rL, double mxL, double myL, double mzL, double g)
+{
+    double r_pos = jl_max(rL, 1e-30);
+    double m2 = mxL * mxL + myL * myL + mzL * mzL;
+    double KE_raw = (0.5 * m2) / r_pos;
+    double KE = jl_min(KE_raw, (0.5 * r_pos) * 1e4);   /* _V_CAP_SQ = 1e4 */
+    return PL / (g - 1.0) + KE;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ghost fills (src/euler3d.jl:33-62), per-axis bc: 0 = outflow, 1 = periodic */
+
+void orc_fill_ghost(double *U, int64_t nx, int64_t ny, int64_t nz, const int bc[3])
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG, Nz = nz + 2 * NG;
+    /* x faces: all (j, k) */
+    for (int64_t k = 0; k < Nz; ++k)
+        for (int64_t j = 0; j < Ny; ++j)
+            for (int g = 0; g < NG; ++g) {
+                int64_t dl = g, dr = NG + nx + g, sl, sr;
+                if (bc[0] == 1) { sl = nx + g; sr = NG + g; }     /* periodic */
+                else            { sl = NG; sr = NG + nx - 1; }    /* outflow */
+                for (int q = 0; q < 5; ++q) {
+                    U[U_IDX(q, dl, j, k, Nx, Ny)] = U[U_IDX(q, sl, j, k, Nx, Ny)];
+                    U[U_IDX(q, dr, j, k, Nx, Ny)] = U[U_IDX(q, sr, j, k, Nx, Ny)];
+                }
+            }
+    /* y faces: all (i, k) */
+    for (int64_t k = 0; k < Nz; ++k)
+        for (int g = 0; g < NG; ++g)
+            for (int64_t i = 0; i < Nx; ++i) {
+                int64_t dl = g, dr = NG + ny + g, sl, sr;
+                if (bc[1] == 1) { sl = ny + g; sr = NG + g; }
+                else            { sl = NG; sr = NG + ny - 1; }
+                for (int q = 0; q < 5; ++q) {
+                    U[U_IDX(q, i, dl, k, Nx, Ny)] = U[U_IDX(q, i, sl, k, Nx, Ny)];
+                    U[U_IDX(q, i, dr, k, Nx, Ny)] = U[U_IDX(q, i, sr, k, Nx, Ny)];
+                }
+            }
+    /* z faces: all (i, j) */
+    for (int g = 0; g < NG; ++g)
+        for (int64_t j = 0; j < Ny; ++j)
+            for (int64_t i = 0; i < Nx; ++i) {
+                int64_t dl = g, dr = NG + nz + g, sl, sr;
+                if (bc[2] == 1) { sl = nz + g; sr = NG + g; }
+                else            { sl = NG; sr = NG + nz - 1; }
+                for (int q = 0; q < 5; ++q) {
+                    U[U_IDX(q, i, j, dl, Nx, Ny)] = U[U_IDX(q, i, j, sl, Nx, Ny)];
+                    U[U_IDX(q, i, j, dr, Nx, Ny)] = U[U_IDX(q, i, j, sr, Nx, Ny)];
+                }
+            }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Floors (src/gpu_kernels.jl:26-46)                                         */
+
+void orc_apply_floors(double *U, int64_t nx, int64_t ny, int64_t nz,
+                      double rho_floor, double P_floor, double g)
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = NG; k < NG + nz; ++k)
+        for (int64_t j = NG; j < NG + ny; ++j)
+            for (int64_t i = NG; i < NG + nx; ++i) {
+                double *u = &U[U_IDX(0, i, j, k, Nx, Ny)];
+                double r = u[0];
+                if (r < rho_floor) {
+                    u[0] = rho_floor; u[1] = 0.0; u[2] = 0.0; u[3] = 0.0;
+                    u[4] = P_floor / (g - 1.0);
+                } else {
+                    double KE = (0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3])) / r;
+                    double P = (g - 1.0) * (u[4] - KE);
+                    if (P < P_floor) u[4] = P_floor / (g - 1.0) + KE;
+                }
+            }
+}
+
+/* ------------------------------------------------------------------------ */
+/* WENO + HLLC fluxes (src/gpu_kernels.jl:51-154)                            */
+/* F? has the reference shape: Fx (5, Nx+1, Ny, Nz), Fy (5, Nx, Ny+1, Nz),
+ * Fz (5, Nx, Ny, Nz+1).  Only the active-range interfaces are written.      */
+
+static void interface_flux(const double *U, int64_t c0, int64_t stride, double g,
+                           int dir, double F[5])
+{
+    /* c0 = flat index (field 0) of the reference's cell "i", the left cell
+     * of the interface; stride = flat distance between neighbours along dir. */
+    double v[5][6];
+    for (int q = 0; q < 5; ++q)
+        for (int s = 0; s < 6; ++s)
+            v[q][s] = U[c0 + (s - 2) * stride + q];
+    double L[5], R[5];
+    for (int q = 0; q < 4; ++q) {
+        L[q] = orc_weno5_left(v[q][0], v[q][1], v[q][2], v[q][3], v[q][4]);
+        R[q] = orc_weno5_right(v[q][1], v[q][2], v[q][3], v[q][4], v[q][5]);
+    }
+    double p[6];
+    for (int s = 0; s < 6; ++s)
+        p[s] = orc_cell_pressure(v[0][s], v[1][s], v[2][s], v[3][s], v[4][s], g);
+    double P_maxL = jl_max(jl_max(jl_max(jl_max(p[0], p[1]), p[2]), p[3]), p[4]);
+    double P_maxR = jl_max(jl_max(jl_max(jl_max(p[1], p[2]), p[3]), p[4]), p[5]);
+    double PL = jl_clamp(orc_weno5_left(p[0], p[1], p[2], p[3], p[4]), 0.0, P_maxL);
+    double PR = jl_clamp(orc_weno5_right(p[1], p[2], p[3], p[4], p[5]), 0.0, P_maxR);
+    L[4] = orc_EL_from_P(PL, L[0], L[1], L[2], L[3], g);
+    R[4] = orc_EL_from_P(PR, R[0], R[1], R[2], R[3], g);
+    orc_hllc_flux(dir, L, R, g, F);
+}
+
+void orc_weno_fluxes(int dir, double *F, const double *U,
+                     int64_t nx, int64_t ny, int64_t nz, double g)
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG;
+    /* flux array extents */
+    const int64_t FNx = Nx + (dir == 0), FNy = Ny + (dir == 1);
+    const int64_t ilo = dir == 0 ? NG - 1 : NG, ihi = NG + nx;   /* exclusive hi */
+    const int64_t jlo = dir == 1 ? NG - 1 : NG, jhi = NG + ny;
+    const int64_t klo = dir == 2 ? NG - 1 : NG, khi = NG + nz;
+    const int64_t stride = dir == 0 ? 5 : dir == 1 ? 5 * Nx : 5 * Nx * Ny;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = klo; k < khi; ++k)
+        for (int64_t j = jlo; j < jhi; ++j)
+            for (int64_t i = ilo; i < ihi; ++i) {
+                double f[5];
+                interface_flux(U, U_IDX(0, i, j, k, Nx, Ny), stride, g, dir, f);
+                for (int q = 0; q < 5; ++q)
+                    F[(int64_t)q + 5 * (i + FNx * (j + FNy * k))] = f[q];
+            }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Flux divergence (src/gpu_kernels.jl:159-169, src/euler3d.jl:135-145)      */
+
+void orc_flux_divergence(double *dU, const double *Fx, const double *Fy, const double *Fz,
+                         int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz)
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG, Nz = nz + 2 * NG;
+    memset(dU, 0, sizeof(double) * 5 * Nx * Ny * Nz);
+    const double idx = 1.0 / dx, idy = 1.0 / dy, idz = 1.0 / dz;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = NG; k < NG + nz; ++k)
+        for (int64_t j = NG; j < NG + ny; ++j)
+            for (int64_t i = NG; i < NG + nx; ++i)
+                for (int q = 0; q < 5; ++q) {
+                    double fx1 = Fx[q + 5 * (i + (Nx + 1) * (j + Ny * k))];
+                    double fx0 = Fx[q + 5 * ((i - 1) + (Nx + 1) * (j + Ny * k))];
+                    double fy1 = Fy[q + 5 * (i + Nx * (j + (Ny + 1) * k))];
+                    double fy0 = Fy[q + 5 * (i + Nx * ((j - 1) + (Ny + 1) * k))];
+                    double fz1 = Fz[q + 5 * (i + Nx * (j + Ny * k))];
+                    double fz0 = Fz[q + 5 * (i + Nx * (j + Ny * (k - 1)))];
+                    dU[U_IDX(q, i, j, k, Nx, Ny)] =
+                        (-(fx1 - fx0)) * idx - (fy1 - fy0) * idy - (fz1 - fz0) * idz;
+                }
+}
+
+/* ------------------------------------------------------------------------ */
+/* phi gravity source (src/gpu_kernels.jl:291-319, src/fmr_gravity.jl:146-156) */
+/* phi is (nx, ny, nz) column-major, active cells only.                        */
+
+void orc_apply_sg_force(double *dU, const double *U, const double *phi,
+                        int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz)
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG;
+    const double i2x = 0.5 / dx, i2y = 0.5 / dy, i2z = 0.5 / dz;
+#define PHI(a, b, c) phi[((a) - 1) + nx * (((b) - 1) + ny * ((c) - 1))]
+#pragma omp parallel for schedule(static)
+    for (int64_t kk = 1; kk <= nz; ++kk)
+        for (int64_t jj = 1; jj <= ny; ++jj)
+            for (int64_t ii = 1; ii <= nx; ++ii) {
+                int64_t i = ii + NG - 1, j = jj + NG - 1, k = kk + NG - 1;
+                const double *u = &U[U_IDX(0, i, j, k, Nx, Ny)];
+                double r = u[0], mx = u[1], my = u[2], mz = u[3];
+                int64_t ip = ii + 1 < nx ? ii + 1 : nx, im = ii - 1 > 1 ? ii - 1 : 1;
+                int64_t jp = jj + 1 < ny ? jj + 1 : ny, jm = jj - 1 > 1 ? jj - 1 : 1;
+                int64_t kp = kk + 1 < nz ? kk + 1 : nz, km = kk - 1 > 1 ? kk - 1 : 1;
+                double fx = ip > im ? i2x : 2.0 * i2x;
+                double fy = jp > jm ? i2y : 2.0 * i2y;
+                double fz = kp > km ? i2z : 2.0 * i2z;
+                double gx = (PHI(ip, jj, kk) - PHI(im, jj, kk)) * fx;
+                double gy = (PHI(ii, jp, kk) - PHI(ii, jm, kk)) * fy;
+                double gz = (PHI(ii, jj, kp) - PHI(ii, jj, km)) * fz;
+                double vdotg = (mx * gx + my * gy + mz * gz) / r;
+                double *d = &dU[U_IDX(0, i, j, k, Nx, Ny)];
+                d[1] -= r * gx;
+                d[2] -= r * gy;
+                d[3] -= r * gz;
+                d[4] -= r * vdotg;
+            }
+#undef PHI
+}
+
+/* ------------------------------------------------------------------------ */
+/* CFL (src/gpu_kernels.jl:174-191, src/euler3d.jl:157-169)                  */
+
+double orc_cfl_dt_3d(const double *U, int64_t nx, int64_t ny, int64_t nz,
+                     double dx, double dy, double dz, double g, double cfl)
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG;
+    const double idx = 1.0 / dx, idy = 1.0 / dy, idz = 1.0 / dz;
+    double smax = 0.0;
+    int any_nan = 0;
+#pragma omp parallel for schedule(static) reduction(max:smax) reduction(|:any_nan)
+    for (int64_t k = NG; k < NG + nz; ++k)
+        for (int64_t j = NG; j < NG + ny; ++j)
+            for (int64_t i = NG; i < NG + nx; ++i) {
+                const double *u = &U[U_IDX(0, i, j, k, Nx, Ny)];
+                double r = jl_max(u[0], 1e-30);
+                double vx = u[1] / r, vy = u[2] / r, vz = u[3] / r;
+                double KE = 0.5 * (vx * vx + vy * vy + vz * vz);
+                double P = jl_max(((g - 1.0) * (u[4] / r - KE)) * r, 0.0);
+                double cs = sqrt((g * P) / r);
+                double s = jl_max(jl_max((fabs(vx) + cs) * idx, (fabs(vy) + cs) * idy),
+                                  (fabs(vz) + cs) * idz);
+                if (isnan(s)) any_nan = 1;
+                else if (s > smax) smax = s;
+            }
+    if (any_nan) return NAN;
+    return smax > 0.0 ? cfl / smax : INFINITY;
+}
+
+/* ------------------------------------------------------------------------ */
+/* RHS and step (src/euler3d.jl:182-260)                                     */
+
+void orc_euler3d_rhs(double *dU, double *Fx, double *Fy, double *Fz, double *U,
+                     int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz,
+                     double g, const int bc[3], double rho_floor, double P_floor,
+                     int fill_ghosts)
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG, Nz = nz + 2 * NG;
+    if (fill_ghosts) orc_fill_ghost(U, nx, ny, nz, bc);
+    orc_apply_floors(U, nx, ny, nz, rho_floor, P_floor, g);
+    memset(Fx, 0, sizeof(double) * 5 * (Nx + 1) * Ny * Nz);
+    memset(Fy, 0, sizeof(double) * 5 * Nx * (Ny + 1) * Nz);
+    memset(Fz, 0, sizeof(double) * 5 * Nx * Ny * (Nz + 1));
+    orc_weno_fluxes(0, Fx, U, nx, ny, nz, g);
+    orc_weno_fluxes(1, Fy, U, nx, ny, nz, g);
+    orc_weno_fluxes(2, Fz, U, nx, ny, nz, g);
+    orc_flux_divergence(dU, Fx, Fy, Fz, nx, ny, nz, dx, dy, dz);
+}
+
+/* Shu-Osher stage combination over the whole array (euler3d.jl:244,250,256;
+ * rk3.jl:24,28,32).  stage is 1, 2 or 3. */
+void orc_rk3_combine(int stage, double *U, const double *Un, const double *dU,
+                     double dt, int64_t n)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < n; ++e) {
+        if (stage == 1)      U[e] = Un[e] + dt * dU[e];
+        else if (stage == 2) U[e] = 0.75 * Un[e] + 0.25 * U[e] + (0.25 * dt) * dU[e];
+        else                 U[e] = (1.0 / 3.0) * Un[e] + (2.0 / 3.0) * U[e] + ((2.0 / 3.0) * dt) * dU[e];
+    }
+}
+
+/* Allocating step: mirrors euler3d_step! with an optional static phi source
+ * (applied after the divergence, before the combine, as fmr3d.jl:509-513). */
+int orc_euler3d_step(double *U, int64_t nx, int64_t ny, int64_t nz,
+                     double dx, double dy, double dz, double dt, double g,
+                     const int bc[3], double rho_floor, double P_floor, const double *phi)
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG, Nz = nz + 2 * NG;
+    const int64_t n = 5 * Nx * Ny * Nz;
+    double *Fx = malloc(sizeof(double) * 5 * (Nx + 1) * Ny * Nz);
+    double *Fy = malloc(sizeof(double) * 5 * Nx * (Ny + 1) * Nz);
+    double *Fz = malloc(sizeof(double) * 5 * Nx * Ny * (Nz + 1));
+    double *dU = malloc(sizeof(double) * n);
+    double *Un = malloc(sizeof(double) * n);
+    if (!Fx || !Fy || !Fz || !dU || !Un) {
+        free(Fx); free(Fy); free(Fz); free(dU); free(Un);
+        return -1;
+    }
+    memcpy(Un, U, sizeof(double) * n);
+    for (int stage = 1; stage <= 3; ++stage) {
+        orc_euler3d_rhs(dU, Fx, Fy, Fz, U, nx, ny, nz, dx, dy, dz, g, bc,
+                        rho_floor, P_floor, 1);
+        if (phi) orc_apply_sg_force(dU, U, phi, nx, ny, nz, dx, dy, dz);
+        orc_rk3_combine(stage, U, Un, dU, dt, n);
+        orc_apply_floors(U, nx, ny, nz, rho_floor, P_floor, g);
+    }
+    free(Fx); free(Fy); free(Fz); free(dU); free(Un);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Conservation totals (src/diagnostics.jl:21-58).  Sums run over active
+ * cells in column-major order with pairwise blocks; the summation order of
+ * Julia's sum is not reproduced, so these agree to rounding only.           */
+
+void orc_conserved_totals(const double *U, int64_t nx, int64_t ny, int64_t nz,
+                          double dV, double out[5])
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG;
+    for (int q = 0; q < 5; ++q) {
+        long double acc = 0.0L;
+        for (int64_t k = NG; k < NG + nz; ++k)
+            for (int64_t j = NG; j < NG + ny; ++j)
+                for (int64_t i = NG; i < NG + nx; ++i)
+                    acc += U[U_IDX(q, i, j, k, Nx, Ny)];
+        out[q] = (double)acc * dV;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Two-level FMR (src/fmr3d.jl)                                              */
+
+void orc_lag5_weights(int ratio, double *w /* ratio x 5, row-major [s][j] */)
+{
+    /* fmr3d.jl:32-46 */
+    static const double nodes[5] = {-2.0, -1.0, 0.0, 1.0, 2.0};
+    for (int s = 0; s < ratio; ++s) {
+        double t = ((double)s + 0.5 - (double)ratio / 2.0) / (double)ratio;
+        for (int j = 0; j < 5; ++j) {
+            double wj = 1.0;
+            for (int m = 0; m < 5; ++m)
+                if (m != j) wj *= (t - nodes[m]) / (nodes[j] - nodes[m]);
+            w[s * 5 + j] = wj;
+        }
+    }
+}
+
+/* fmr3d.jl:165-170; ig_f is the reference's 1-based fine index. */
+static void coarse_stencil(int64_t ig_f, int64_t c_lo, int ratio, int64_t *ic0, int *sx)
+{
+    double xi = ((double)c_lo - 0.5) + ((double)(ig_f - NG) - 0.5) / (double)ratio;
+    double r = rint(xi);                 /* Julia round: nearest, ties to even */
+    *ic0 = (int64_t)r;
+    double f = floor(((xi - r) + 0.5) * (double)ratio);
+    int s = (int)f;
+    *sx = s < 0 ? 0 : (s > ratio - 1 ? ratio - 1 : s);
+}
+
+typedef struct {
+    int64_t ncx, ncy, ncz, nfx, nfy, nfz;
+    int64_t ci_lo, ci_hi, cj_lo, cj_hi, ck_lo, ck_hi;   /* 1-based coarse active */
+    int ratio;
+    double w[4 * 5];
+} fmr_geom;
+
+/* fmr3d.jl:176-198; indices 1-based as in the reference */
+static void prolong_cell(double *Uf, const double *Uc, const fmr_geom *G,
+                         int64_t ig_f, int64_t jf, int64_t kf)
+{
+    const int64_t Ncx = G->ncx + 2 * NG, Ncy = G->ncy + 2 * NG;
+    const int64_t Nfx = G->nfx + 2 * NG, Nfy = G->nfy + 2 * NG;
+    int64_t ic0, jc0, kc0; int si, sj, sk;
+    coarse_stencil(ig_f, G->ci_lo, G->ratio, &ic0, &si);
+    coarse_stencil(jf, G->cj_lo, G->ratio, &jc0, &sj);
+    coarse_stencil(kf, G->ck_lo, G->ratio, &kc0, &sk);
+    for (int q = 0; q < 5; ++q) {
+        double val = 0.0;
+        for (int di = -2; di <= 2; ++di) {
+            int64_t ig_c = NG + ic0 + di;
+            double wi = G->w[si * 5 + di + 2];
+            for (int dj = -2; dj <= 2; ++dj) {
+                int64_t jg_c = NG + jc0 + dj;
+                double wij = wi * G->w[sj * 5 + dj + 2];
+                for (int dk = -2; dk <= 2; ++dk) {
+                    int64_t kg_c = NG + kc0 + dk;
+                    val += (wij * G->w[sk * 5 + dk + 2]) *
+                           Uc[U_IDX(q, ig_c - 1, jg_c - 1, kg_c - 1, Ncx, Ncy)];
+                }
+            }
+        }
+        Uf[U_IDX(q, ig_f - 1, jf - 1, kf - 1, Nfx, Nfy)] = val;
+    }
+}
+
+static void fill_geom(fmr_geom *G, int64_t ncx, int64_t ncy, int64_t ncz,
+                      const int64_t lo_hi[6], int ratio)
+{
+    G->ncx = ncx; G->ncy = ncy; G->ncz = ncz;
+    G->ci_lo = lo_hi[0]; G->ci_hi = lo_hi[1];
+    G->cj_lo = lo_hi[2]; G->cj_hi = lo_hi[3];
+    G->ck_lo = lo_hi[4]; G->ck_hi = lo_hi[5];
+    G->ratio = ratio;
+    G->nfx = (lo_hi[1] - lo_hi[0] + 1) * ratio;
+    G->nfy = (lo_hi[3] - lo_hi[2] + 1) * ratio;
+    G->nfz = (lo_hi[5] - lo_hi[4] + 1) * ratio;
+    orc_lag5_weights(ratio, G->w);
+}
+
+void orc_prolong_all(double *Uf, const double *Uc, int64_t ncx, int64_t ncy, int64_t ncz,
+                     const int64_t lo_hi[6], int ratio)
+{
+    fmr_geom G; fill_geom(&G, ncx, ncy, ncz, lo_hi, ratio);
+    const int64_t Nfx = G.nfx + 2 * NG, Nfy = G.nfy + 2 * NG, Nfz = G.nfz + 2 * NG;
+#pragma omp parallel for schedule(static)
+    for (int64_t kf = 1; kf <= Nfz; ++kf)
+        for (int64_t jf = 1; jf <= Nfy; ++jf)
+            for (int64_t ig = 1; ig <= Nfx; ++ig)
+                prolong_cell(Uf, Uc, &G, ig, jf, kf);
+}
+
+/* fmr3d.jl:223-255 */
+void orc_prolong_fine_ghosts(double *Uf, const double *Uc, int64_t ncx, int64_t ncy,
+                             int64_t ncz, const int64_t lo_hi[6], int ratio)
+{
+    fmr_geom G; fill_geom(&G, ncx, ncy, ncz, lo_hi, ratio);
+    const int64_t nxf = G.nfx, nyf = G.nfy, nzf = G.nfz;
+    const int64_t Nfx = nxf + 2 * NG, Nfy = nyf + 2 * NG, Nfz = nzf + 2 * NG;
+#pragma omp parallel for schedule(static)
+    for (int64_t kf = 1; kf <= Nfz; ++kf)
+        for (int64_t jf = 1; jf <= Nfy; ++jf) {
+            for (int64_t ig = 1; ig <= NG; ++ig) prolong_cell(Uf, Uc, &G, ig, jf, kf);
+            for (int64_t ig = NG + nxf + 1; ig <= Nfx; ++ig) prolong_cell(Uf, Uc, &G, ig, jf, kf);
+        }
+#pragma omp parallel for schedule(static)
+    for (int64_t kf = 1; kf <= Nfz; ++kf)
+        for (int64_t ig = NG + 1; ig <= NG + nxf; ++ig) {
+            for (int64_t jf = 1; jf <= NG; ++jf) prolong_cell(Uf, Uc, &G, ig, jf, kf);
+            for (int64_t jf = NG + nyf + 1; jf <= Nfy; ++jf) prolong_cell(Uf, Uc, &G, ig, jf, kf);
+        }
+#pragma omp parallel for schedule(static)
+    for (int64_t jf = NG + 1; jf <= NG + nyf; ++jf)
+        for (int64_t ig = NG + 1; ig <= NG + nxf; ++ig) {
+            for (int64_t kf = 1; kf <= NG; ++kf) prolong_cell(Uf, Uc, &G, ig, jf, kf);
+            for (int64_t kf = NG + nzf + 1; kf <= Nfz; ++kf) prolong_cell(Uf, Uc, &G, ig, jf, kf);
+        }
+}
+
+/* fmr3d.jl:266-285 */
+void orc_restrict(double *Uc, const double *Uf, int64_t ncx, int64_t ncy, int64_t ncz,
+                  const int64_t lo_hi[6], int ratio)
+{
+    fmr_geom G; fill_geom(&G, ncx, ncy, ncz, lo_hi, ratio);
+    const int64_t Ncx = ncx + 2 * NG, Ncy = ncy + 2 * NG;
+    const int64_t Nfx = G.nfx + 2 * NG, Nfy = G.nfy + 2 * NG;
+    const int r = ratio;
+    const double inv_r3 = 1.0 / (double)(r * r * r);
+    for (int64_t m = 0; m <= G.ci_hi - G.ci_lo; ++m)
+        for (int64_t n = 0; n <= G.cj_hi - G.cj_lo; ++n)
+            for (int64_t p = 0; p <= G.ck_hi - G.ck_lo; ++p) {
+                int64_t ic = NG + G.ci_lo + m, jc = NG + G.cj_lo + n, kc = NG + G.ck_lo + p;
+                for (int q = 0; q < 5; ++q) {
+                    double val = 0.0;
+                    for (int sx = 1; sx <= r; ++sx)
+                        for (int sy = 1; sy <= r; ++sy)
+                            for (int sz = 1; sz <= r; ++sz)
+                                val += Uf[U_IDX(q, NG + r * m + sx - 1, NG + r * n + sy - 1,
+                                                NG + r * p + sz - 1, Nfx, Nfy)];
+                    Uc[U_IDX(q, ic - 1, jc - 1, kc - 1, Ncx, Ncy)] = val * inv_r3;
+                }
+            }
+}
+
+/* Flux registers (fmr3d.jl:81-96): six faces, each (5, t1, t2) column-major. */
+typedef struct { double *f[6]; int64_t n1[6], n2[6]; } fluxreg;
+
+static int fr_alloc(fluxreg *fr, int64_t nic, int64_t njc, int64_t nkc)
+{
+    int64_t a[6][2] = {{njc, nkc}, {njc, nkc}, {nic, nkc}, {nic, nkc}, {nic, njc}, {nic, njc}};
+    for (int f = 0; f < 6; ++f) {
+        fr->n1[f] = a[f][0]; fr->n2[f] = a[f][1];
+        fr->f[f] = calloc((size_t)(5 * a[f][0] * a[f][1]), sizeof(double));
+        if (!fr->f[f]) return -1;
+    }
+    return 0;
+}
+static void fr_free(fluxreg *fr) { for (int f = 0; f < 6; ++f) free(fr->f[f]); }
+#define FR(fr, face, q, a, b) (fr)->f[face][(q) + 5 * (((a) - 1) + (fr)->n1[face] * ((b) - 1))]
+
+/* fmr3d.jl:292-326 */
+static void accumulate_fr_coarse(fluxreg *fr, const double *Fx, const double *Fy,
+                                 const double *Fz, double weight, double dt, const fmr_geom *G)
+{
+    const double fac = weight * dt;
+    const int64_t Nx = G->ncx + 2 * NG, Ny = G->ncy + 2 * NG;
+#define FXC(q, i, j, k) Fx[(q) + 5 * (((i) - 1) + (Nx + 1) * (((j) - 1) + Ny * ((k) - 1)))]
+#define FYC(q, i, j, k) Fy[(q) + 5 * (((i) - 1) + Nx * (((j) - 1) + (Ny + 1) * ((k) - 1)))]
+#define FZC(q, i, j, k) Fz[(q) + 5 * (((i) - 1) + Nx * (((j) - 1) + Ny * ((k) - 1)))]
+    for (int64_t kc = G->ck_lo; kc <= G->ck_hi; ++kc)
+        for (int64_t jc = G->cj_lo; jc <= G->cj_hi; ++jc)
+            for (int q = 0; q < 5; ++q) {
+                FR(fr, 0, q, jc - G->cj_lo + 1, kc - G->ck_lo + 1) += fac * FXC(q, NG + G->ci_lo - 1, NG + jc, NG + kc);
+                FR(fr, 1, q, jc - G->cj_lo + 1, kc - G->ck_lo + 1) += fac * FXC(q, NG + G->ci_hi, NG + jc, NG + kc);
+            }
+    for (int64_t kc = G->ck_lo; kc <= G->ck_hi; ++kc)
+        for (int64_t ic = G->ci_lo; ic <= G->ci_hi; ++ic)
+            for (int q = 0; q < 5; ++q) {
+                FR(fr, 2, q, ic - G->ci_lo + 1, kc - G->ck_lo + 1) += fac * FYC(q, NG + ic, NG + G->cj_lo - 1, NG + kc);
+                FR(fr, 3, q, ic - G->ci_lo + 1, kc - G->ck_lo + 1) += fac * FYC(q, NG + ic, NG + G->cj_hi, NG + kc);
+            }
+    for (int64_t jc = G->cj_lo; jc <= G->cj_hi; ++jc)
+        for (int64_t ic = G->ci_lo; ic <= G->ci_hi; ++ic)
+            for (int q = 0; q < 5; ++q) {
+                FR(fr, 4, q, ic - G->ci_lo + 1, jc - G->cj_lo + 1) += fac * FZC(q, NG + ic, NG + jc, NG + G->ck_lo - 1);
+                FR(fr, 5, q, ic - G->ci_lo + 1, jc - G->cj_lo + 1) += fac * FZC(q, NG + ic, NG + jc, NG + G->ck_hi);
+            }
+#undef FXC
+#undef FYC
+#undef FZC
+}
+
+/* fmr3d.jl:331-369 */
+static void accumulate_fr_fine(fluxreg *fr, const double *Fx, const double *Fy,
+                               const double *Fz, double weight, double dt, const fmr_geom *G)
+{
+    const int64_t r = G->ratio;
+    const double fac = (weight * dt) / (double)(r * r);
+    const int64_t nxf = G->nfx, nyf = G->nfy, nzf = G->nfz;
+    const int64_t Nx = nxf + 2 * NG, Ny = nyf + 2 * NG;
+#define FXF(q, i, j, k) Fx[(q) + 5 * (((i) - 1) + (Nx + 1) * (((j) - 1) + Ny * ((k) - 1)))]
+#define FYF(q, i, j, k) Fy[(q) + 5 * (((i) - 1) + Nx * (((j) - 1) + (Ny + 1) * ((k) - 1)))]
+#define FZF(q, i, j, k) Fz[(q) + 5 * (((i) - 1) + Nx * (((j) - 1) + Ny * ((k) - 1)))]
+    for (int64_t kf = 1; kf <= nzf; ++kf)
+        for (int64_t jf = 1; jf <= nyf; ++jf) {
+            int64_t kc = (kf - 1) / r + 1, jc = (jf - 1) / r + 1;
+            for (int q = 0; q < 5; ++q) {
+                FR(fr, 0, q, jc, kc) += fac * FXF(q, NG, NG + jf, NG + kf);
+                FR(fr, 1, q, jc, kc) += fac * FXF(q, NG + nxf, NG + jf, NG + kf);
+            }
+        }
+    for (int64_t kf = 1; kf <= nzf; ++kf)
+        for (int64_t ia = 1; ia <= nxf; ++ia) {
+            int64_t kc = (kf - 1) / r + 1, ic = (ia - 1) / r + 1;
+            for (int q = 0; q < 5; ++q) {
+                FR(fr, 2, q, ic, kc) += fac * FYF(q, NG + ia, NG, NG + kf);
+                FR(fr, 3, q, ic, kc) += fac * FYF(q, NG + ia, NG + nyf, NG + kf);
+            }
+        }
+    for (int64_t jf = 1; jf <= nyf; ++jf)
+        for (int64_t ia = 1; ia <= nxf; ++ia) {
+            int64_t jc = (jf - 1) / r + 1, ic = (ia - 1) / r + 1;
+            for (int q = 0; q < 5; ++q) {
+                FR(fr, 4, q, ic, jc) += fac * FZF(q, NG + ia, NG + jf, NG);
+                FR(fr, 5, q, ic, jc) += fac * FZF(q, NG + ia, NG + jf, NG + nzf);
+            }
+        }
+#undef FXF
+#undef FYF
+#undef FZF
+}
+
+/* fmr3d.jl:379-422 */
+static void bc_correct(double *Uc, const fluxreg *frc, const fluxreg *frf,
+                       const fmr_geom *G, double dx, double dy, double dz)
+{
+    const int64_t Nx = G->ncx + 2 * NG, Ny = G->ncy + 2 * NG;
+    const double idx = 1.0 / dx, idy = 1.0 / dy, idz = 1.0 / dz;
+    const int64_t nic = G->ci_hi - G->ci_lo + 1, njc = G->cj_hi - G->cj_lo + 1,
+                  nkc = G->ck_hi - G->ck_lo + 1;
+#define UC(q, i, j, k) Uc[U_IDX(q, (i) - 1, (j) - 1, (k) - 1, Nx, Ny)]
+    for (int64_t b = 1; b <= nkc; ++b)
+        for (int64_t a = 1; a <= njc; ++a) {
+            int64_t jc = G->cj_lo + a - 1, kc = G->ck_lo + b - 1;
+            for (int q = 0; q < 5; ++q) {
+                double dlo = (FR(frc, 0, q, a, b) - FR(frf, 0, q, a, b)) * idx;
+                double dhi = (FR(frf, 1, q, a, b) - FR(frc, 1, q, a, b)) * idx;
+                UC(q, NG + G->ci_lo - 1, NG + jc, NG + kc) += dlo;
+                UC(q, NG + G->ci_hi + 1, NG + jc, NG + kc) += dhi;
+            }
+        }
+    for (int64_t b = 1; b <= nkc; ++b)
+        for (int64_t a = 1; a <= nic; ++a) {
+            int64_t ic = G->ci_lo + a - 1, kc = G->ck_lo + b - 1;
+            for (int q = 0; q < 5; ++q) {
+                double dlo = (FR(frc, 2, q, a, b) - FR(frf, 2, q, a, b)) * idy;
+                double dhi = (FR(frf, 3, q, a, b) - FR(frc, 3, q, a, b)) * idy;
+                UC(q, NG + ic, NG + G->cj_lo - 1, NG + kc) += dlo;
+                UC(q, NG + ic, NG + G->cj_hi + 1, NG + kc) += dhi;
+            }
+        }
+    for (int64_t b = 1; b <= njc; ++b)
+        for (int64_t a = 1; a <= nic; ++a) {
+            int64_t ic = G->ci_lo + a - 1, jc = G->cj_lo + b - 1;
+            for (int q = 0; q < 5; ++q) {
+                double dlo = (FR(frc, 4, q, a, b) - FR(frf, 4, q, a, b)) * idz;
+                double dhi = (FR(frf, 5, q, a, b) - FR(frc, 5, q, a, b)) * idz;
+                UC(q, NG + ic, NG + jc, NG + G->ck_lo - 1) += dlo;
+                UC(q, NG + ic, NG + jc, NG + G->ck_hi + 1) += dhi;
+            }
+        }
+#undef UC
+}
+
+/* fine-level RHS (fmr3d.jl:427-436): no ghost fill */
+static void fine_rhs(double *dU, double *Fx, double *Fy, double *Fz, double *U,
+                     int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz,
+                     double g, double rho_floor, double P_floor)
+{
+    static const int bc_unused[3] = {0, 0, 0};
+    orc_euler3d_rhs(dU, Fx, Fy, Fz, U, nx, ny, nz, dx, dy, dz, g, bc_unused,
+                    rho_floor, P_floor, 0);
+}
+
+/* fmr3d.jl:451-604 with a frozen phi per level (phi_c, phi_f may be NULL). */
+int orc_fmr3d_step(double *Uc, double *Uf,
+                   int64_t ncx, int64_t ncy, int64_t ncz, double dxc, double dyc, double dzc,
+                   const int64_t lo_hi[6], int ratio, double g, const int bc[3],
+                   double rho_floor, double P_floor, double dt_c,
+                   const double *phi_c, const double *phi_f)
+{
+    fmr_geom G; fill_geom(&G, ncx, ncy, ncz, lo_hi, ratio);
+    const int r = ratio;
+    const double dt_f = dt_c / (double)r;
+    const double dxf = dxc / r, dyf = dyc / r, dzf = dzc / r;
+    const int64_t Ncx = ncx + 2 * NG, Ncy = ncy + 2 * NG, Ncz = ncz + 2 * NG;
+    const int64_t nfx = G.nfx, nfy = G.nfy, nfz = G.nfz;
+    const int64_t Nfx = nfx + 2 * NG, Nfy = nfy + 2 * NG, Nfz = nfz + 2 * NG;
+    const int64_t nc = 5 * Ncx * Ncy * Ncz, nf = 5 * Nfx * Nfy * Nfz;
+    int rc = -1;
+
+    double *Fxc = calloc((size_t)(5 * (Ncx + 1) * Ncy * Ncz), 8);
+    double *Fyc = calloc((size_t)(5 * Ncx * (Ncy + 1) * Ncz), 8);
+    double *Fzc = calloc((size_t)(5 * Ncx * Ncy * (Ncz + 1)), 8);
+    double *dUc = calloc((size_t)nc, 8);
+    double *Fxf = calloc((size_t)(5 * (Nfx + 1) * Nfy * Nfz), 8);
+    double *Fyf = calloc((size_t)(5 * Nfx * (Nfy + 1) * Nfz), 8);
+    double *Fzf = calloc((size_t)(5 * Nfx * Nfy * (Nfz + 1)), 8);
+    double *dUf = calloc((size_t)nf, 8);
+    double *Ucn = malloc((size_t)nc * 8), *Ucnp1 = malloc((size_t)nc * 8);
+    double *Ucint = malloc((size_t)nc * 8), *Unf = malloc((size_t)nf * 8);
+    fluxreg frc, frf;
+    memset(&frc, 0, sizeof frc); memset(&frf, 0, sizeof frf);
+    const int64_t nic = G.ci_hi - G.ci_lo + 1, njc = G.cj_hi - G.cj_lo + 1,
+                  nkc = G.ck_hi - G.ck_lo + 1;
+    if (!Fxc || !Fyc || !Fzc || !dUc || !Fxf || !Fyf || !Fzf || !dUf || !Ucn ||
+        !Ucnp1 || !Ucint || !Unf || fr_alloc(&frc, nic, njc, nkc) || fr_alloc(&frf, nic, njc, nkc))
+        goto done;
+
+    const double rk_w[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};   /* fmr3d.jl:496 */
+
+    /* coarse SSP-RK3 (fmr3d.jl:501-538); Un_c == U_c_n (both copies of U_c) */
+    memcpy(Ucn, Uc, (size_t)nc * 8);
+    for (int s = 1; s <= 3; ++s) {
+        orc_euler3d_rhs(dUc, Fxc, Fyc, Fzc, Uc, ncx, ncy, ncz, dxc, dyc, dzc, g, bc,
+                        rho_floor, P_floor, 1);
+        accumulate_fr_coarse(&frc, Fxc, Fyc, Fzc, rk_w[s - 1], dt_c, &G);
+        if (phi_c) orc_apply_sg_force(dUc, Uc, phi_c, ncx, ncy, ncz, dxc, dyc, dzc);
+        orc_rk3_combine(s, Uc, Ucn, dUc, dt_c, nc);
+        orc_apply_floors(Uc, ncx, ncy, ncz, rho_floor, P_floor, g);
+    }
+    memcpy(Ucnp1, Uc, (size_t)nc * 8);
+
+    /* fine sub-cycles (fmr3d.jl:547-589) */
+    for (int ss = 0; ss < r; ++ss) {
+        double alpha = (double)ss / (double)r;
+        for (int64_t e = 0; e < nc; ++e)
+            Ucint[e] = (1.0 - alpha) * Ucn[e] + alpha * Ucnp1[e];
+        memcpy(Unf, Uf, (size_t)nf * 8);
+        for (int s = 1; s <= 3; ++s) {
+            orc_prolong_fine_ghosts(Uf, Ucint, ncx, ncy, ncz, lo_hi, ratio);
+            fine_rhs(dUf, Fxf, Fyf, Fzf, Uf, nfx, nfy, nfz, dxf, dyf, dzf, g, rho_floor, P_floor);
+            accumulate_fr_fine(&frf, Fxf, Fyf, Fzf, rk_w[s - 1], dt_f, &G);
+            if (phi_f) orc_apply_sg_force(dUf, Uf, phi_f, nfx, nfy, nfz, dxf, dyf, dzf);
+            orc_rk3_combine(s, Uf, Unf, dUf, dt_f, nf);
+            orc_apply_floors(Uf, nfx, nfy, nfz, rho_floor, P_floor, g);
+        }
+    }
+
+    orc_restrict(Uc, Uf, ncx, ncy, ncz, lo_hi, ratio);                 /* fmr3d.jl:593 */
+    bc_correct(Uc, &frc, &frf, &G, dxc, dyc, dzc);                     /* fmr3d.jl:597 */
+    orc_apply_floors(Uc, ncx, ncy, ncz, rho_floor, P_floor, g);        /* fmr3d.jl:601 */
+    rc = 0;
+done:
+    free(Fxc); free(Fyc); free(Fzc); free(dUc); free(Fxf); free(Fyf); free(Fzf); free(dUf);
+    free(Ucn); free(Ucnp1); free(Ucint); free(Unf);
+    fr_free(&frc); fr_free(&frf);
+    return rc;
+}
+
+/* fmr3d.jl:616-622 */
+double orc_cfl_dt_fmr3d(const double *Uc, const double *Uf,
+                        int64_t ncx, int64_t ncy, int64_t ncz, double dxc, double dyc, double dzc,
+                        const int64_t lo_hi[6], int ratio, double g, double cfl)
+{
+    int64_t nfx = (lo_hi[1] - lo_hi[0] + 1) * ratio, nfy = (lo_hi[3] - lo_hi[2] + 1) * ratio,
+            nfz = (lo_hi[5] - lo_hi[4] + 1) * ratio;
+    double dtc = orc_cfl_dt_3d(Uc, ncx, ncy, ncz, dxc, dyc, dzc, g, cfl);
+    double dtf = orc_cfl_dt_3d(Uf, nfx, nfy, nfz, dxc / ratio, dyc / ratio, dzc / ratio, g, cfl);
+    return jl_min(dtc, (double)ratio * dtf);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sub-cycled multi-level step: the two-level fmr3d_step! applied recursively.
+ * Level l+1 is nested in level l with ratio[l] and box lo_hi[6*l..6*l+5]
+ * (1-based active indices of level l).  With nlev == 2 this is exactly
+ * orc_fmr3d_step.  phis[l] may be NULL.                                        */
+
+static int level_subcycle(int l, int nlev, double **U, const int64_t *dims,
+                          const double *dxs, const int64_t *lo_hi, const int *ratio,
+                          double g, const int bc[3], double rho_floor, double P_floor,
+                          double dt, const double *const *phis, fluxreg *fr_parent,
+                          const fmr_geom *G_parent, const double *Uc_n, const double *Uc_np1);
+
+static int level_rk3(int l, int nlev, double **U, const int64_t *dims, const double *dxs,
+                     const int64_t *lo_hi, const int *ratio, double g, const int bc[3],
+                     double rho_floor, double P_floor, double dt,
+                     const double *const *phis, fluxreg *fr_own, const fmr_geom *G_own,
+                     fluxreg *fr_parent, const fmr_geom *G_parent, const double *Ucint)
+{
+    /* One SSP-RK3 step of level l.  A root level fills its ghosts by bc; a
+     * nested level fills them by prolongation from its parent's Ucint.  Fluxes
+     * through this level's child box go to fr_own (coarse side); fluxes
+     * through this level's own outer faces go to fr_parent (fine side). */
+    const int64_t nx = dims[3 * l], ny = dims[3 * l + 1], nz = dims[3 * l + 2];
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG, Nz = nz + 2 * NG;
+    const int64_t n = 5 * Nx * Ny * Nz;
+    const double dx = dxs[3 * l], dy = dxs[3 * l + 1], dz = dxs[3 * l + 2];
+    const double rk_w[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+    double *Fx = calloc((size_t)(5 * (Nx + 1) * Ny * Nz), 8);
+    double *Fy = calloc((size_t)(5 * Nx * (Ny + 1) * Nz), 8);
+    double *Fz = calloc((size_t)(5 * Nx * Ny * (Nz + 1)), 8);
+    double *dU = calloc((size_t)n, 8), *Un = malloc((size_t)n * 8);
+    int rc = -1;
+    if (!Fx || !Fy || !Fz || !dU || !Un) goto done;
+    memcpy(Un, U[l], (size_t)n * 8);
+    for (int s = 1; s <= 3; ++s) {
+        if (l == 0) {
+            orc_euler3d_rhs(dU, Fx, Fy, Fz, U[l], nx, ny, nz, dx, dy, dz, g, bc,
+                            rho_floor, P_floor, 1);
+        } else {
+            const int64_t *lh = &lo_hi[6 * (l - 1)];
+            orc_prolong_fine_ghosts(U[l], Ucint, dims[3 * (l - 1)], dims[3 * (l - 1) + 1],
+                                    dims[3 * (l - 1) + 2], lh, ratio[l - 1]);
+            fine_rhs(dU, Fx, Fy, Fz, U[l], nx, ny, nz, dx, dy, dz, g, rho_floor, P_floor);
+        }
+        if (fr_own) accumulate_fr_coarse(fr_own, Fx, Fy, Fz, rk_w[s - 1], dt, G_own);
+        if (fr_parent) accumulate_fr_fine(fr_parent, Fx, Fy, Fz, rk_w[s - 1], dt, G_parent);
+        if (phis && phis[l]) orc_apply_sg_force(dU, U[l], phis[l], nx, ny, nz, dx, dy, dz);
+        orc_rk3_combine(s, U[l], Un, dU, dt, n);
+        orc_apply_floors(U[l], nx, ny, nz, rho_floor, P_floor, g);
+    }
+    rc = 0;
+done:
+    free(Fx); free(Fy); free(Fz); free(dU); free(Un);
+    return rc;
+}
+
+static int level_advance(int l, int nlev, double **U, const int64_t *dims, const double *dxs,
+                         const int64_t *lo_hi, const int *ratio, double g, const int bc[3],
+                         double rho_floor, double P_floor, double dt,
+                         const double *const *phis, fluxreg *fr_parent,
+                         const fmr_geom *G_parent, const double *Ucint)
+{
+    /* Advance level l by dt: own RK3, then (if it has a child) the child's
+     * sub-cycles, restriction and Berger-Colella correction. */
+    const int64_t nx = dims[3 * l], ny = dims[3 * l + 1], nz = dims[3 * l + 2];
+    const int64_t n = 5 * (nx + 2 * NG) * (ny + 2 * NG) * (nz + 2 * NG);
+    if (l == nlev - 1)
+        return level_rk3(l, nlev, U, dims, dxs, lo_hi, ratio, g, bc, rho_floor, P_floor, dt,
+                         phis, NULL, NULL, fr_parent, G_parent, Ucint);
+    fmr_geom G; fill_geom(&G, nx, ny, nz, &lo_hi[6 * l], ratio[l]);
+    fluxreg frc, frf;
+    memset(&frc, 0, sizeof frc); memset(&frf, 0, sizeof frf);
+    double *Un = malloc((size_t)n * 8), *Unp1 = malloc((size_t)n * 8), *Uint = malloc((size_t)n * 8);
+    int rc = -1;
+    const int64_t nic = G.ci_hi - G.ci_lo + 1, njc = G.cj_hi - G.cj_lo + 1, nkc = G.ck_hi - G.ck_lo + 1;
+    if (!Un || !Unp1 || !Uint || fr_alloc(&frc, nic, njc, nkc) || fr_alloc(&frf, nic, njc, nkc))
+        goto done;
+    memcpy(Un, U[l], (size_t)n * 8);
+    if (level_rk3(l, nlev, U, dims, dxs, lo_hi, ratio, g, bc, rho_floor, P_floor, dt, phis,
+                  &frc, &G, fr_parent, G_parent, Ucint))
+        goto done;
+    memcpy(Unp1, U[l], (size_t)n * 8);
+    rc = level_subcycle(l + 1, nlev, U, dims, dxs, lo_hi, ratio, g, bc, rho_floor, P_floor,
+                        dt / (double)ratio[l], phis, &frf, &G, Un, Unp1);
+    if (rc) goto done;
+    orc_restrict(U[l], U[l + 1], nx, ny, nz, &lo_hi[6 * l], ratio[l]);
+    bc_correct(U[l], &frc, &frf, &G, dxs[3 * l], dxs[3 * l + 1], dxs[3 * l + 2]);
+    orc_apply_floors(U[l], nx, ny, nz, rho_floor, P_floor, g);
+    rc = 0;
+done:
+    free(Un); free(Unp1); free(Uint);
+    fr_free(&frc); fr_free(&frf);
+    return rc;
+}
+
+static int level_subcycle(int l, int nlev, double **U, const int64_t *dims,
+                          const double *dxs, const int64_t *lo_hi, const int *ratio,
+                          double g, const int bc[3], double rho_floor, double P_floor,
+                          double dt, const double *const *phis, fluxreg *fr_parent,
+                          const fmr_geom *G_parent, const double *Uc_n, const double *Uc_np1)
+{
+    const int64_t pn = 5 * (dims[3 * (l - 1)] + 2 * NG) * (dims[3 * (l - 1) + 1] + 2 * NG) *
+                       (dims[3 * (l - 1) + 2] + 2 * NG);
+    const int r = ratio[l - 1];
+    double *Ucint = malloc((size_t)pn * 8);
+    if (!Ucint) return -1;
+    int rc = 0;
+    for (int ss = 0; ss < r && !rc; ++ss) {
+        double alpha = (double)ss / (double)r;
+        for (int64_t e = 0; e < pn; ++e)
+            Ucint[e] = (1.0 - alpha) * Uc_n[e] + alpha * Uc_np1[e];
+        rc = level_advance(l, nlev, U, dims, dxs, lo_hi, ratio, g, bc, rho_floor, P_floor, dt,
+                           phis, fr_parent, G_parent, Ucint);
+    }
+    free(Ucint);
+    return rc;
+}
+
+int orc_fmr_multilevel_step(int nlev, double **U, const int64_t *dims, const double *dxs,
+                            const int64_t *lo_hi, const int *ratio, double g, const int bc[3],
+                            double rho_floor, double P_floor, double dt,
+                            const double *const *phis)
+{
+    if (nlev < 1) return -1;
+    return level_advance(0, nlev, U, dims, dxs, lo_hi, ratio, g, bc, rho_floor, P_floor, dt,
+                         phis, NULL, NULL, NULL);
+}
