@@ -1,0 +1,1173 @@
+// bsn_api.cpp -- host orchestration and the C ABI of libbsn_hip (include/bsn.h).
+//
+// A "plan" is a set of grids of one precision with device-resident tile
+// work-lists for every kernel; a single grid, a batch of independent problems
+// and each FMR level all run through plans.  One SSP-RK3 step of a plan is
+//   stage 1: ghost fill (saving the raw state to Un) -> floors -> x/y/z fluxes
+//            -> fused divergence + phi source + combine + floors
+//   stage 2, 3: ghost fill -> fluxes -> fused update
+//   then the stage-3 combination of ghost cells,
+// which is src/euler3d.jl:220-260 with the RHS of src/euler3d.jl:182-205.  The
+// start-of-stage floors of stages 2 and 3 are omitted: they follow the floors
+// applied after the previous combination on the same active cells, and the
+// floor is idempotent (a second application rewrites E to the same value).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "bsn_internal.hpp"
+#include "bsn_launch.hpp"
+#include "../../include/bsn.h"
+
+using namespace bsn;
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return set_err(BSN_ERR_HIP, "%s failed: %s (%s:%d)", #expr,                 \
+                           hipGetErrorString(_e), __FILE__, __LINE__);                  \
+    } while (0)
+
+#define CHK(expr)                       \
+    do {                                \
+        int _r = (expr);                \
+        if (_r != BSN_OK) return _r;    \
+    } while (0)
+
+static inline int64_t round_up(int64_t a, int64_t m) { return (a + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------
+// Objects
+
+struct bsn_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 1;
+    char name[256] = {0};
+    LaunchCtx L() const { return LaunchCtx{stream, cus}; }
+};
+
+struct Plan;
+
+struct bsn_grid {
+    bsn_ctx* ctx = nullptr;
+    GridDev h{};               // host copy of the descriptor
+    bool params_set = false;
+    double dx = 1, dy = 1, dz = 1;
+    double* d_phi = nullptr;
+    double* d_scratch = nullptr;   // dU-sized scratch (U layout)
+    double* d_stage = nullptr;     // reference-layout staging
+    size_t stage_bytes = 0;
+    double* d_part = nullptr;      // totals partial sums
+    Plan* plan = nullptr;          // single-grid plan
+};
+
+struct Plan {
+    bsn_ctx* ctx = nullptr;
+    std::vector<bsn_grid*> grids;
+    int prec = 0;
+    GridDev* d_grids = nullptr;
+    int shape[3] = {0, 0, 0};
+    Tile* d_ft[3] = {nullptr, nullptr, nullptr};
+    int nft[3] = {0, 0, 0};
+    Tile* d_ut = nullptr;
+    int nut = 0;
+    int64_t* d_goff = nullptr;
+    int64_t gtotal = 0;
+    int64_t* d_aoff = nullptr;
+    int64_t atotal = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    // profiling events (optional)
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_kind;   // 0..2 flux x/y/z, 3 update
+};
+
+struct bsn_batch {
+    Plan* plan = nullptr;
+};
+
+struct bsn_fmr {
+    bsn_ctx* ctx = nullptr;
+    int nlev = 0;
+    std::vector<bsn_grid*> lev;
+    std::vector<Plan*> plans;
+    GridDev* d_all = nullptr;
+    std::vector<FmrDev> hf;
+    FmrDev* d_f = nullptr;
+    std::vector<double> alpha;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// Tile lists
+
+static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int shape) {
+    int W, H;
+    flux_shape_dims(dir, shape, W, H);
+    std::vector<Tile> t;
+    for (int g = 0; g < (int)G.size(); ++g) {
+        const GridDev& d = G[g];
+        if (dir == 0) {
+            for (int k = 0; k < d.nz; ++k) {
+                for (int j0 = 0; j0 < d.ny; j0 += H)
+                    for (int f0 = 0; f0 < d.nx; f0 += W) t.push_back({g, f0, j0, k});
+                for (int j0 = 0; j0 < d.ny; j0 += W * H) t.push_back({g, -1, j0, k});
+            }
+        } else if (dir == 1) {
+            for (int k = 0; k < d.nz; ++k)
+                for (int i0 = 0; i0 < d.nx; i0 += W)
+                    for (int f0 = 0; f0 <= d.ny; f0 += H) t.push_back({g, f0, i0, k});
+        } else {
+            for (int j = 0; j < d.ny; ++j)
+                for (int i0 = 0; i0 < d.nx; i0 += W)
+                    for (int f0 = 0; f0 <= d.nz; f0 += H) t.push_back({g, f0, i0, j});
+        }
+    }
+    return t;
+}
+
+static std::vector<Tile> update_tiles(const std::vector<GridDev>& G) {
+    std::vector<Tile> t;
+    for (int g = 0; g < (int)G.size(); ++g) {
+        const GridDev& d = G[g];
+        for (int k0 = 0; k0 < d.nz; k0 += UPDATE_KZ)
+            for (int j0 = 0; j0 < d.ny; j0 += 4)
+                for (int i0 = 0; i0 < d.nx; i0 += 64) t.push_back({g, i0, j0, k0});
+    }
+    return t;
+}
+
+template <typename T>
+static int upload_vec(T** dptr, const std::vector<T>& v) {
+    if (*dptr) { HIPCHK(hipFree(*dptr)); *dptr = nullptr; }
+    if (v.empty()) return BSN_OK;
+    HIPCHK(hipMalloc((void**)dptr, v.size() * sizeof(T)));
+    HIPCHK(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return BSN_OK;
+}
+
+static std::vector<GridDev> host_descs(const Plan& P) {
+    std::vector<GridDev> v;
+    for (auto* g : P.grids) v.push_back(g->h);
+    return v;
+}
+
+static void graph_drop(hipGraphExec_t& e, hipGraph_t& g) {
+    if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+    if (g) { (void)hipGraphDestroy(g); g = nullptr; }
+}
+
+static int plan_set_shape(Plan& P, int dir, int shape) {
+    auto G = host_descs(P);
+    auto t = flux_tiles(G, dir, shape);
+    CHK(upload_vec(&P.d_ft[dir], t));
+    P.nft[dir] = (int)t.size();
+    P.shape[dir] = shape;
+    graph_drop(P.gexec, P.graph);
+    return BSN_OK;
+}
+
+static int plan_refresh(Plan& P) {
+    HIPCHK(hipSetDevice(P.ctx->device));
+    auto G = host_descs(P);
+    if (!P.d_grids) HIPCHK(hipMalloc((void**)&P.d_grids, G.size() * sizeof(GridDev)));
+    HIPCHK(hipMemcpy(P.d_grids, G.data(), G.size() * sizeof(GridDev), hipMemcpyHostToDevice));
+    return BSN_OK;
+}
+
+static int plan_build(Plan& P) {
+    HIPCHK(hipSetDevice(P.ctx->device));
+    CHK(plan_refresh(P));
+    auto G = host_descs(P);
+    for (int d = 0; d < 3; ++d) CHK(plan_set_shape(P, d, P.shape[d]));
+    auto ut = update_tiles(G);
+    CHK(upload_vec(&P.d_ut, ut));
+    P.nut = (int)ut.size();
+    std::vector<int64_t> go(G.size() + 1, 0), ao(G.size() + 1, 0);
+    for (size_t g = 0; g < G.size(); ++g) {
+        const GridDev& d = G[g];
+        go[g + 1] = go[g] + 2 * BSN_NG * ((int64_t)d.Ny * d.Nz + (int64_t)d.nx * d.Nz + (int64_t)d.nx * d.ny);
+        ao[g + 1] = ao[g] + (int64_t)d.nx * d.ny * d.nz;
+    }
+    CHK(upload_vec(&P.d_goff, go));
+    CHK(upload_vec(&P.d_aoff, ao));
+    P.gtotal = go.back();
+    P.atotal = ao.back();
+    return BSN_OK;
+}
+
+static void plan_free(Plan* P) {
+    if (!P) return;
+    (void)hipSetDevice(P->ctx->device);
+    graph_drop(P->gexec, P->graph);
+    for (auto e : P->ev) (void)hipEventDestroy(e);
+    (void)hipFree(P->d_grids);
+    for (int d = 0; d < 3; ++d) (void)hipFree(P->d_ft[d]);
+    (void)hipFree(P->d_ut);
+    (void)hipFree(P->d_goff);
+    (void)hipFree(P->d_aoff);
+    delete P;
+}
+
+static int plan_new(bsn_ctx* ctx, const std::vector<bsn_grid*>& grids, Plan** out) {
+    if (grids.empty()) return set_err(BSN_ERR_ARG, "plan with no grids");
+    Plan* P = new Plan();
+    P->ctx = ctx;
+    P->grids = grids;
+    P->prec = grids[0]->h.prec;
+    for (auto* g : grids) {
+        if (g->h.prec != P->prec) { delete P; return set_err(BSN_ERR_ARG, "grids of mixed precision"); }
+        if (g->ctx != ctx) { delete P; return set_err(BSN_ERR_ARG, "grids of different contexts"); }
+    }
+    int r = plan_build(*P);
+    if (r != BSN_OK) { plan_free(P); return r; }
+    *out = P;
+    return BSN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plan stages
+
+static void prof_mark(Plan& P, int kind) {
+    if (!P.prof) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, P.ctx->stream);
+    P.ev.push_back(e);
+    P.ev_kind.push_back(kind);
+}
+
+static int plan_fluxes(Plan& P) {
+    const LaunchCtx L = P.ctx->L();
+    for (int d = 0; d < 3; ++d) {
+        prof_mark(P, d);
+        HIPCHK(launch_flux(L, d, P.prec, P.shape[d], P.d_grids, P.d_ft[d], P.nft[d], 0));
+    }
+    prof_mark(P, 9);
+    return BSN_OK;
+}
+
+static int plan_update(Plan& P, int stage, int do_cfl, double* dU_out = nullptr) {
+    prof_mark(P, 3);
+    HIPCHK(launch_update(P.ctx->L(), P.prec, P.d_grids, P.d_ut, P.nut, stage, do_cfl, dU_out));
+    prof_mark(P, 9);
+    return BSN_OK;
+}
+
+static int plan_ghost_fill(Plan& P, int copy_un) {
+    HIPCHK(launch_ghost_fill(P.ctx->L(), P.d_grids, P.d_goff, (int)P.grids.size(), P.gtotal, copy_un));
+    return BSN_OK;
+}
+
+static int plan_floors(Plan& P, int copy_un) {
+    HIPCHK(launch_floors(P.ctx->L(), P.d_grids, P.d_aoff, (int)P.grids.size(), P.atotal, copy_un));
+    return BSN_OK;
+}
+
+static int plan_ghost_combine(Plan& P, int stage) {
+    HIPCHK(launch_ghost_combine(P.ctx->L(), P.d_grids, P.d_goff, (int)P.grids.size(), P.gtotal, stage));
+    return BSN_OK;
+}
+
+// One SSP-RK3 step with bc ghost fills; dt is read from each grid's scal.
+static int plan_step(Plan& P, int fused_cfl) {
+    for (int s = 1; s <= 3; ++s) {
+        if (s == 1) {
+            CHK(plan_ghost_fill(P, 1));
+            CHK(plan_floors(P, 1));
+        } else {
+            CHK(plan_ghost_fill(P, 0));
+        }
+        CHK(plan_fluxes(P));
+        CHK(plan_update(P, s, fused_cfl && s == 3));
+    }
+    CHK(plan_ghost_combine(P, 3));
+    return BSN_OK;
+}
+
+static int plan_cfl(Plan& P, double cfl, double t_end) {
+    const LaunchCtx L = P.ctx->L();
+    for (auto* g : P.grids) {
+        HIPCHK(launch_fill_u64(L, (uint64_t*)&g->h.scal[SCAL_SMAX], 1, 0));
+        HIPCHK(launch_set_scal(L, g->h.scal, SCAL_CFL, cfl));
+        HIPCHK(launch_set_scal(L, g->h.scal, SCAL_TEND, t_end));
+    }
+    HIPCHK(launch_cfl(L, P.d_grids, P.d_ut, P.nut));
+    return BSN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Library / context
+
+extern "C" int bsn_version(void) { return 10000; }
+extern "C" const char* bsn_last_error(void) { return g_err.c_str(); }
+
+extern "C" int bsn_ctx_create(int device, bsn_ctx** out) {
+    if (!out) return set_err(BSN_ERR_ARG, "out is NULL");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_err(BSN_ERR_ARG, "device %d of %d", device, n);
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    bsn_ctx* c = new bsn_ctx();
+    c->device = device;
+    c->cus = prop.multiProcessorCount;
+    snprintf(c->name, sizeof c->name, "%s (%s), %d CUs, %zu MB", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount, (size_t)(prop.totalGlobalMem >> 20));
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return set_err(BSN_ERR_HIP, "stream: %s", hipGetErrorString(e)); }
+    *out = c;
+    return BSN_OK;
+}
+
+extern "C" int bsn_ctx_destroy(bsn_ctx* c) {
+    if (!c) return BSN_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BSN_OK;
+}
+
+extern "C" int bsn_ctx_synchronize(bsn_ctx* c) {
+    if (!c) return set_err(BSN_ERR_ARG, "ctx is NULL");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BSN_OK;
+}
+
+extern "C" int bsn_ctx_info(bsn_ctx* c, char* buf, int len) {
+    if (!c || !buf || len <= 0) return set_err(BSN_ERR_ARG, "bad arguments");
+    snprintf(buf, (size_t)len, "%s", c->name);
+    return BSN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grids
+
+extern "C" int bsn_grid_create(bsn_ctx* ctx, int64_t nx, int64_t ny, int64_t nz, int precision,
+                               bsn_grid** out) {
+    if (!ctx || !out) return set_err(BSN_ERR_ARG, "NULL argument");
+    if (nx < 1 || ny < 1 || nz < 1 || nx > (1 << 20) || ny > (1 << 20) || nz > (1 << 20))
+        return set_err(BSN_ERR_ARG, "bad grid dims %lld x %lld x %lld", (long long)nx,
+                       (long long)ny, (long long)nz);
+    if (precision != BSN_PREC_FP64 && precision != BSN_PREC_SPLIT)
+        return set_err(BSN_ERR_ARG, "precision %d", precision);
+    HIPCHK(hipSetDevice(ctx->device));
+    bsn_grid* g = new bsn_grid();
+    g->ctx = ctx;
+    GridDev& h = g->h;
+    h.nx = (int)nx; h.ny = (int)ny; h.nz = (int)nz;
+    h.Ny = h.ny + 2 * BSN_NG;
+    h.Nz = h.nz + 2 * BSN_NG;
+    const int Nx = h.nx + 2 * BSN_NG;
+    h.pitch = (int)round_up(BSN_XOFF + Nx, 16);
+    h.fs = round_up((int64_t)h.pitch * h.Ny * h.Nz, 64);
+    h.prec = precision;
+    const int64_t esz = precision == BSN_PREC_FP64 ? 8 : 4;
+    h.fp[0] = (int)round_up(h.nx + 1, 32);
+    h.fp[1] = (int)round_up(h.nx, 32);
+    h.fp[2] = (int)round_up(h.nx, 32);
+    h.ffs[0] = round_up((int64_t)h.fp[0] * h.ny * h.nz, 128);
+    h.ffs[1] = round_up((int64_t)h.fp[1] * (h.ny + 1) * h.nz, 128);
+    h.ffs[2] = round_up((int64_t)h.fp[2] * h.ny * (h.nz + 1), 128);
+    h.gamma = 1.4; h.gfac = (h.gamma + 1.0) / (2.0 * h.gamma);
+    h.idx = h.idy = h.idz = 1.0;
+    h.bc = 0;
+    auto fail = [&](hipError_t e) {
+        (void)hipFree(h.U); (void)hipFree(h.Un); (void)hipFree(h.scal);
+        for (int d = 0; d < 3; ++d) (void)hipFree(h.F[d]);
+        delete g;
+        return set_err(BSN_ERR_HIP, "allocation failed: %s", hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc((void**)&h.U, 5 * h.fs * 8)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc((void**)&h.Un, 5 * h.fs * 8)) != hipSuccess) return fail(e);
+    for (int d = 0; d < 3; ++d)
+        if ((e = hipMalloc(&h.F[d], 5 * h.ffs[d] * esz)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc((void**)&h.scal, SCAL_COUNT * 8)) != hipSuccess) return fail(e);
+    // padding and never-written flux entries stay defined
+    (void)hipMemset(h.U, 0, 5 * h.fs * 8);
+    (void)hipMemset(h.Un, 0, 5 * h.fs * 8);
+    for (int d = 0; d < 3; ++d) (void)hipMemset(h.F[d], 0, 5 * h.ffs[d] * esz);
+    (void)hipMemset(h.scal, 0, SCAL_COUNT * 8);
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e);
+    *out = g;
+    return BSN_OK;
+}
+
+extern "C" int bsn_grid_destroy(bsn_grid* g) {
+    if (!g) return BSN_OK;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    plan_free(g->plan);
+    GridDev& h = g->h;
+    (void)hipFree(h.U); (void)hipFree(h.Un); (void)hipFree(h.scal);
+    for (int d = 0; d < 3; ++d) (void)hipFree(h.F[d]);
+    (void)hipFree(g->d_phi);
+    (void)hipFree(g->d_scratch);
+    (void)hipFree(g->d_stage);
+    (void)hipFree(g->d_part);
+    delete g;
+    return BSN_OK;
+}
+
+static int64_t ref_count(const GridDev& h) {
+    return 5LL * (h.nx + 2 * BSN_NG) * h.Ny * h.Nz;
+}
+
+static int ensure_stage(bsn_grid* g, size_t bytes) {
+    if (g->stage_bytes >= bytes) return BSN_OK;
+    if (g->d_stage) HIPCHK(hipFree(g->d_stage));
+    g->d_stage = nullptr;
+    g->stage_bytes = 0;
+    HIPCHK(hipMalloc((void**)&g->d_stage, bytes));
+    g->stage_bytes = bytes;
+    return BSN_OK;
+}
+
+static int ensure_scratch(bsn_grid* g) {
+    if (g->d_scratch) return BSN_OK;
+    HIPCHK(hipMalloc((void**)&g->d_scratch, 5 * g->h.fs * 8));
+    HIPCHK(hipMemset(g->d_scratch, 0, 5 * g->h.fs * 8));
+    return BSN_OK;
+}
+
+static int grid_plan(bsn_grid* g) {
+    if (g->plan) return BSN_OK;
+    return plan_new(g->ctx, {g}, &g->plan);
+}
+
+static int sync_ctx(bsn_ctx* c) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BSN_OK;
+}
+
+// reference layout (host or device) -> device buffer with U's layout
+static int import_ref(bsn_grid* g, const double* src, int where, double* dst) {
+    const LaunchCtx L = g->ctx->L();
+    const size_t bytes = (size_t)ref_count(g->h) * 8;
+    const double* from = src;
+    if (where == BSN_HOST) {
+        CHK(ensure_stage(g, bytes));
+        HIPCHK(hipMemcpyAsync(g->d_stage, src, bytes, hipMemcpyHostToDevice, g->ctx->stream));
+        from = g->d_stage;
+    }
+    HIPCHK(launch_aos_to_soa(L, g->h, from, dst));
+    return BSN_OK;
+}
+
+static int export_ref(bsn_grid* g, const double* src, double* dst, int where) {
+    const LaunchCtx L = g->ctx->L();
+    const size_t bytes = (size_t)ref_count(g->h) * 8;
+    if (where == BSN_HOST) {
+        CHK(ensure_stage(g, bytes));
+        HIPCHK(launch_soa_to_aos(L, g->h, src, g->d_stage));
+        HIPCHK(hipMemcpyAsync(dst, g->d_stage, bytes, hipMemcpyDeviceToHost, g->ctx->stream));
+    } else {
+        HIPCHK(launch_soa_to_aos(L, g->h, src, dst));
+    }
+    return BSN_OK;
+}
+
+extern "C" int bsn_grid_set_state(bsn_grid* g, const double* U, int where) {
+    if (!g || !U) return set_err(BSN_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(import_ref(g, U, where, g->h.U));
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_grid_get_state(bsn_grid* g, double* U, int where) {
+    if (!g || !U) return set_err(BSN_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(export_ref(g, g->h.U, U, where));
+    return sync_ctx(g->ctx);
+}
+
+static int grid_changed(bsn_grid* g) {
+    // propagate descriptor changes to every plan that holds this grid
+    if (g->plan) CHK(plan_refresh(*g->plan));
+    return BSN_OK;
+}
+
+extern "C" int bsn_grid_set_phi(bsn_grid* g, const double* phi, int where) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    HIPCHK(hipStreamSynchronize(g->ctx->stream));
+    if (!phi) {
+        if (g->d_phi) HIPCHK(hipFree(g->d_phi));
+        g->d_phi = nullptr;
+        g->h.phi = nullptr;
+        return grid_changed(g);
+    }
+    const size_t bytes = (size_t)g->h.nx * g->h.ny * g->h.nz * 8;
+    if (!g->d_phi) HIPCHK(hipMalloc((void**)&g->d_phi, bytes));
+    HIPCHK(hipMemcpy(g->d_phi, phi, bytes, where == BSN_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice));
+    g->h.phi = g->d_phi;
+    return grid_changed(g);
+}
+
+static int apply_params(bsn_grid* g, double dx, double dy, double dz, double gamma, int bc,
+                        double rho_floor, double P_floor) {
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0)) return set_err(BSN_ERR_ARG, "cell sizes must be > 0");
+    if (!(gamma > 1.0)) return set_err(BSN_ERR_ARG, "gamma must be > 1");
+    if (bc < 0 || bc > 7) return set_err(BSN_ERR_ARG, "bc mask %d", bc);
+    for (int a = 0; a < 3; ++a) {
+        const int n = a == 0 ? g->h.nx : a == 1 ? g->h.ny : g->h.nz;
+        if ((bc >> a & 1) && n < BSN_NG)
+            return set_err(BSN_ERR_ARG, "periodic axis %d needs >= %d cells", a, BSN_NG);
+    }
+    GridDev& h = g->h;
+    const double idx = 1.0 / dx, idy = 1.0 / dy, idz = 1.0 / dz;
+    const double gfac = (gamma + 1.0) / (2.0 * gamma);
+    const bool same = g->params_set && h.idx == idx && h.idy == idy && h.idz == idz &&
+                      h.gamma == gamma && h.bc == bc && h.rho_floor == rho_floor &&
+                      h.P_floor == P_floor;
+    g->dx = dx; g->dy = dy; g->dz = dz;
+    h.idx = idx; h.idy = idy; h.idz = idz;
+    h.gamma = gamma; h.gfac = gfac;
+    h.bc = bc;
+    h.rho_floor = rho_floor;
+    h.P_floor = P_floor;
+    g->params_set = true;
+    if (!same) CHK(grid_changed(g));
+    return BSN_OK;
+}
+
+extern "C" int bsn_grid_set_params(bsn_grid* g, double dx, double dy, double dz, double gamma,
+                                   int bc, double rho_floor, double P_floor) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    return apply_params(g, dx, dy, dz, gamma, bc, rho_floor, P_floor);
+}
+
+extern "C" int bsn_grid_device_state(bsn_grid* g, double** U, int64_t* fs, int64_t* pitch) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    if (U) *U = g->h.U;
+    if (fs) *fs = g->h.fs;
+    if (pitch) *pitch = g->h.pitch;
+    return BSN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Uniform-grid drop-ins
+
+extern "C" int bsn_fill_ghost_3d(bsn_grid* g, int bc) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(grid_plan(g));
+    if (bc < 0 || bc > 7) return set_err(BSN_ERR_ARG, "bc mask %d", bc);
+    if (g->h.bc != bc) { g->h.bc = bc; CHK(grid_changed(g)); }
+    CHK(plan_ghost_fill(*g->plan, 0));
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_apply_floors_3d(bsn_grid* g, double rho_floor, double P_floor, double gamma) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(grid_plan(g));
+    CHK(apply_params(g, g->dx, g->dy, g->dz, gamma, g->h.bc, rho_floor, P_floor));
+    CHK(plan_floors(*g->plan, 0));
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_euler3d_rhs(bsn_grid* g, double* dU, double* Fx, double* Fy, double* Fz,
+                               int where, double dx, double dy, double dz, double gamma, int bc,
+                               double rho_floor, double P_floor, int fill_ghosts) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(grid_plan(g));
+    CHK(apply_params(g, dx, dy, dz, gamma, bc, rho_floor, P_floor));
+    Plan& P = *g->plan;
+    const LaunchCtx L = g->ctx->L();
+    if (fill_ghosts) CHK(plan_ghost_fill(P, 0));
+    CHK(plan_floors(P, 0));
+    CHK(plan_fluxes(P));
+    double* F[3] = {Fx, Fy, Fz};
+    for (int d = 0; d < 3; ++d) {
+        if (!F[d]) continue;
+        const GridDev& h = g->h;
+        const size_t n = 5ULL * (h.nx + 2 * BSN_NG + (d == 0)) * (h.Ny + (d == 1)) * (h.Nz + (d == 2));
+        if (where == BSN_HOST) {
+            CHK(ensure_stage(g, n * 8));
+            HIPCHK(launch_flux_export(L, h, d, g->d_stage));
+            HIPCHK(hipMemcpyAsync(F[d], g->d_stage, n * 8, hipMemcpyDeviceToHost, g->ctx->stream));
+            HIPCHK(hipStreamSynchronize(g->ctx->stream));
+        } else {
+            HIPCHK(launch_flux_export(L, h, d, F[d]));
+        }
+    }
+    if (dU) {
+        CHK(ensure_scratch(g));
+        HIPCHK(launch_fill_u64(L, (uint64_t*)g->d_scratch, 5 * g->h.fs, 0));
+        CHK(plan_update(P, 0, 0, g->d_scratch));
+        CHK(export_ref(g, g->d_scratch, dU, where));
+    }
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_euler3d_step(bsn_grid* g, double dx, double dy, double dz, double dt,
+                                double gamma, int bc, double rho_floor, double P_floor) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(grid_plan(g));
+    CHK(apply_params(g, dx, dy, dz, gamma, bc, rho_floor, P_floor));
+    HIPCHK(launch_set_scal(g->ctx->L(), g->h.scal, SCAL_DT, dt));
+    CHK(plan_step(*g->plan, 0));
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_cfl_dt_3d(bsn_grid* g, double dx, double dy, double dz, double gamma,
+                             double cfl, double* dt_out) {
+    if (!g || !dt_out) return set_err(BSN_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(grid_plan(g));
+    CHK(apply_params(g, dx, dy, dz, gamma, g->h.bc, g->h.rho_floor, g->h.P_floor));
+    Plan& P = *g->plan;
+    CHK(plan_cfl(P, cfl, 0.0));
+    HIPCHK(launch_dt(g->ctx->L(), P.d_grids, 1));
+    HIPCHK(hipMemcpyAsync(dt_out, &g->h.scal[SCAL_DT], 8, hipMemcpyDeviceToHost, g->ctx->stream));
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_apply_sg_force(bsn_grid* g, double* dU, int where, double dx, double dy,
+                                  double dz) {
+    if (!g || !dU) return set_err(BSN_ERR_ARG, "NULL argument");
+    if (!g->h.phi) return set_err(BSN_ERR_STATE, "grid has no phi");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(grid_plan(g));
+    GridDev h = g->h;
+    h.idx = 1.0 / dx; h.idy = 1.0 / dy; h.idz = 1.0 / dz;
+    CHK(ensure_scratch(g));
+    CHK(import_ref(g, dU, where, g->d_scratch));
+    HIPCHK(launch_sg_force(g->ctx->L(), h, g->d_scratch));
+    CHK(export_ref(g, g->d_scratch, dU, where));
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_rk3_save(bsn_grid* g) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    HIPCHK(launch_copy_state(g->ctx->L(), g->h, g->h.U, g->h.Un));
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_rk3_combine(bsn_grid* g, int stage, const double* dU, int where, double dt) {
+    if (!g || !dU) return set_err(BSN_ERR_ARG, "NULL argument");
+    if (stage < 1 || stage > 3) return set_err(BSN_ERR_ARG, "stage %d", stage);
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(ensure_scratch(g));
+    CHK(import_ref(g, dU, where, g->d_scratch));
+    HIPCHK(launch_combine_all(g->ctx->L(), g->h, stage, g->d_scratch, dt));
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_conserved_totals(bsn_grid* g, double dV, double out[5]) {
+    if (!g || !out) return set_err(BSN_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    const int nb = 1024;
+    if (!g->d_part) HIPCHK(hipMalloc((void**)&g->d_part, nb * 5 * 8));
+    HIPCHK(launch_totals(g->ctx->L(), g->h, g->d_part, nb, dV, &g->h.scal[SCAL_TOT]));
+    HIPCHK(hipMemcpyAsync(out, &g->h.scal[SCAL_TOT], 5 * 8, hipMemcpyDeviceToHost, g->ctx->stream));
+    return sync_ctx(g->ctx);
+}
+
+// Host-array drop-ins with a per-thread cached context and grid.
+static thread_local bsn_ctx* t_ctx = nullptr;
+static thread_local bsn_grid* t_grid = nullptr;
+
+static int host_grid(int64_t nx, int64_t ny, int64_t nz, int prec) {
+    if (!t_ctx) CHK(bsn_ctx_create(0, &t_ctx));
+    if (t_grid && (t_grid->h.nx != nx || t_grid->h.ny != ny || t_grid->h.nz != nz || t_grid->h.prec != prec)) {
+        bsn_grid_destroy(t_grid);
+        t_grid = nullptr;
+    }
+    if (!t_grid) CHK(bsn_grid_create(t_ctx, nx, ny, nz, prec, &t_grid));
+    return BSN_OK;
+}
+
+extern "C" int bsn_euler3d_step_host(double* U, int64_t nx, int64_t ny, int64_t nz, double dx,
+                                     double dy, double dz, double dt, double gamma, int bc,
+                                     double rho_floor, double P_floor, int precision) {
+    CHK(host_grid(nx, ny, nz, precision));
+    CHK(bsn_grid_set_state(t_grid, U, BSN_HOST));
+    CHK(bsn_euler3d_step(t_grid, dx, dy, dz, dt, gamma, bc, rho_floor, P_floor));
+    return bsn_grid_get_state(t_grid, U, BSN_HOST);
+}
+
+extern "C" int bsn_cfl_dt_3d_host(const double* U, int64_t nx, int64_t ny, int64_t nz, double dx,
+                                  double dy, double dz, double gamma, double cfl, double* dt_out) {
+    CHK(host_grid(nx, ny, nz, t_grid ? t_grid->h.prec : 0));
+    CHK(bsn_grid_set_state(t_grid, U, BSN_HOST));
+    return bsn_cfl_dt_3d(t_grid, dx, dy, dz, gamma, cfl, dt_out);
+}
+
+// ---------------------------------------------------------------------------
+// Batches
+
+extern "C" int bsn_batch_create(bsn_ctx* ctx, int ngrids, bsn_grid* const* grids, bsn_batch** out) {
+    if (!ctx || !grids || !out || ngrids < 1) return set_err(BSN_ERR_ARG, "bad arguments");
+    std::vector<bsn_grid*> v(grids, grids + ngrids);
+    for (auto* g : v)
+        if (!g || !g->params_set) return set_err(BSN_ERR_STATE, "grid without parameters");
+    HIPCHK(hipSetDevice(ctx->device));
+    bsn_batch* b = new bsn_batch();
+    int r = plan_new(ctx, v, &b->plan);
+    if (r != BSN_OK) { delete b; return r; }
+    *out = b;
+    return BSN_OK;
+}
+
+extern "C" int bsn_batch_destroy(bsn_batch* b) {
+    if (!b) return BSN_OK;
+    plan_free(b->plan);
+    delete b;
+    return BSN_OK;
+}
+
+extern "C" int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz) {
+    if (!b) return set_err(BSN_ERR_ARG, "NULL batch");
+    const int s[3] = {sx, sy, sz};
+    for (int d = 0; d < 3; ++d)
+        if (s[d] < 0 || s[d] >= flux_shape_count(d)) return set_err(BSN_ERR_ARG, "shape %d", s[d]);
+    HIPCHK(hipSetDevice(b->plan->ctx->device));
+    for (int d = 0; d < 3; ++d) CHK(plan_set_shape(*b->plan, d, s[d]));
+    return BSN_OK;
+}
+
+extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
+    if (!b) return set_err(BSN_ERR_ARG, "NULL batch");
+    Plan& P = *b->plan;
+    HIPCHK(hipSetDevice(P.ctx->device));
+    const LaunchCtx L = P.ctx->L();
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    // the flux kernels only read U and write F: safe to repeat on the live state
+    CHK(plan_ghost_fill(P, 0));
+    std::string rep = "{";
+    for (int d = 0; d < 3; ++d) {
+        int best = 0;
+        float best_ms = 1e30f;
+        char buf[160];
+        rep += (d ? ", " : "");
+        rep += "\"" + std::string(1, "xyz"[d]) + "\": {";
+        for (int s = 0; s < flux_shape_count(d); ++s) {
+            CHK(plan_set_shape(P, d, s));
+            HIPCHK(launch_flux(L, d, P.prec, s, P.d_grids, P.d_ft[d], P.nft[d], 0));
+            HIPCHK(hipEventRecord(e0, L.stream));
+            const int reps = 3;
+            for (int r = 0; r < reps; ++r)
+                HIPCHK(launch_flux(L, d, P.prec, s, P.d_grids, P.d_ft[d], P.nft[d], 0));
+            HIPCHK(hipEventRecord(e1, L.stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            int W, H;
+            flux_shape_dims(d, s, W, H);
+            snprintf(buf, sizeof buf, "%s\"%dx%d\": %.4f", s ? ", " : "", W, H, ms);
+            rep += buf;
+            if (ms < best_ms) { best_ms = ms; best = s; }
+        }
+        CHK(plan_set_shape(P, d, best));
+        int W, H;
+        flux_shape_dims(d, best, W, H);
+        snprintf(buf, sizeof buf, ", \"best\": \"%dx%d\"}", W, H);
+        rep += buf;
+    }
+    rep += "}";
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (report && report_len > 0) snprintf(report, (size_t)report_len, "%s", rep.c_str());
+    return BSN_OK;
+}
+
+extern "C" int bsn_batch_step(bsn_batch* b, const double* dt) {
+    if (!b || !dt) return set_err(BSN_ERR_ARG, "NULL argument");
+    Plan& P = *b->plan;
+    HIPCHK(hipSetDevice(P.ctx->device));
+    CHK(plan_refresh(P));
+    for (size_t g = 0; g < P.grids.size(); ++g)
+        HIPCHK(launch_set_scal(P.ctx->L(), P.grids[g]->h.scal, SCAL_DT, dt[g]));
+    CHK(plan_step(P, 0));
+    return sync_ctx(P.ctx);
+}
+
+// One device-dt step: dt from the running maximum, RK3 with the stage-3
+// update accumulating the next maximum, then t += dt.
+static int plan_device_step(Plan& P) {
+    const LaunchCtx L = P.ctx->L();
+    HIPCHK(launch_dt(L, P.d_grids, (int)P.grids.size()));
+    CHK(plan_step(P, 1));
+    HIPCHK(launch_advance_t(L, P.d_grids, (int)P.grids.size()));
+    return BSN_OK;
+}
+
+static int plan_run(Plan& P, double cfl, double t_end, int nsteps, int use_graph) {
+    const LaunchCtx L = P.ctx->L();
+    for (auto* g : P.grids) HIPCHK(launch_set_scal(L, g->h.scal, SCAL_T, 0.0));
+    CHK(plan_cfl(P, cfl, t_end));
+    if (!use_graph) {
+        for (int s = 0; s < nsteps; ++s) CHK(plan_device_step(P));
+        return sync_ctx(P.ctx);
+    }
+    if (!P.gexec) {
+        HIPCHK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+        int r = plan_device_step(P);
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(L.stream, &gr);
+        if (r != BSN_OK) { if (gr) (void)hipGraphDestroy(gr); return r; }
+        if (e != hipSuccess) return set_err(BSN_ERR_HIP, "capture: %s", hipGetErrorString(e));
+        P.graph = gr;
+        HIPCHK(hipGraphInstantiate(&P.gexec, P.graph, nullptr, nullptr, 0));
+    }
+    for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(P.gexec, L.stream));
+    return sync_ctx(P.ctx);
+}
+
+extern "C" int bsn_batch_run(bsn_batch* b, double cfl, double t_end, int nsteps, int use_graph) {
+    if (!b || nsteps < 0) return set_err(BSN_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(b->plan->ctx->device));
+    CHK(plan_refresh(*b->plan));
+    return plan_run(*b->plan, cfl, t_end, nsteps, use_graph);
+}
+
+extern "C" int bsn_batch_times(bsn_batch* b, double* t, double* dt) {
+    if (!b) return set_err(BSN_ERR_ARG, "NULL batch");
+    Plan& P = *b->plan;
+    HIPCHK(hipSetDevice(P.ctx->device));
+    for (size_t g = 0; g < P.grids.size(); ++g) {
+        double s[SCAL_COUNT];
+        HIPCHK(hipMemcpy(s, P.grids[g]->h.scal, sizeof s, hipMemcpyDeviceToHost));
+        if (t) t[g] = s[SCAL_T];
+        if (dt) dt[g] = s[SCAL_DT];
+    }
+    return BSN_OK;
+}
+
+extern "C" int bsn_batch_profile(bsn_batch* b, int nsteps, double* ms_flux3, double* ms_update,
+                                 double* ms_step) {
+    if (!b || nsteps < 1) return set_err(BSN_ERR_ARG, "bad arguments");
+    Plan& P = *b->plan;
+    HIPCHK(hipSetDevice(P.ctx->device));
+    const LaunchCtx L = P.ctx->L();
+    hipEvent_t s0, s1;
+    HIPCHK(hipEventCreate(&s0));
+    HIPCHK(hipEventCreate(&s1));
+    P.prof = true;
+    HIPCHK(hipEventRecord(s0, L.stream));
+    int r = BSN_OK;
+    for (int s = 0; s < nsteps && r == BSN_OK; ++s) r = plan_device_step(P);
+    P.prof = false;
+    HIPCHK(hipEventRecord(s1, L.stream));
+    HIPCHK(hipEventSynchronize(s1));
+    double acc[4] = {0, 0, 0, 0};
+    int cnt[4] = {0, 0, 0, 0};
+    for (size_t e = 0; e + 1 < P.ev.size(); ++e) {
+        const int k = P.ev_kind[e];
+        if (k > 3) continue;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, P.ev[e], P.ev[e + 1]) == hipSuccess) { acc[k] += ms; cnt[k]++; }
+    }
+    for (auto e : P.ev) (void)hipEventDestroy(e);
+    P.ev.clear();
+    P.ev_kind.clear();
+    float tot = 0;
+    HIPCHK(hipEventElapsedTime(&tot, s0, s1));
+    (void)hipEventDestroy(s0);
+    (void)hipEventDestroy(s1);
+    if (r != BSN_OK) return r;
+    if (ms_flux3) for (int d = 0; d < 3; ++d) ms_flux3[d] = cnt[d] ? acc[d] / cnt[d] : 0.0;
+    if (ms_update) *ms_update = cnt[3] ? acc[3] / cnt[3] : 0.0;
+    if (ms_step) *ms_step = tot / nsteps;
+    return BSN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FMR
+
+// nearest coarse active index of a fine index (_coarse_stencil, fmr3d.jl:165-170)
+static int coarse_centre(int ig_f1, int c_lo, int ratio) {
+    const double xi = ((double)c_lo - 0.5) + ((double)(ig_f1 - BSN_NG) - 0.5) / (double)ratio;
+    return (int)rint(xi);
+}
+
+extern "C" int bsn_fmr_create(bsn_ctx* ctx, int nlevels, bsn_grid* const* levels,
+                              const int64_t* lo_hi, const int* ratio, bsn_fmr** out) {
+    if (!ctx || !levels || !out || nlevels < 1 || nlevels > BSN_MAXLEV)
+        return set_err(BSN_ERR_ARG, "bad arguments");
+    if (nlevels > 1 && (!lo_hi || !ratio)) return set_err(BSN_ERR_ARG, "missing boxes");
+    HIPCHK(hipSetDevice(ctx->device));
+    for (int l = 0; l < nlevels; ++l) {
+        if (!levels[l] || !levels[l]->params_set) return set_err(BSN_ERR_STATE, "level %d without parameters", l);
+        if (levels[l]->h.prec != levels[0]->h.prec) return set_err(BSN_ERR_ARG, "mixed precision");
+    }
+    for (int l = 0; l + 1 < nlevels; ++l) {
+        const GridDev& P = levels[l]->h;
+        const GridDev& C = levels[l + 1]->h;
+        const int64_t* b = &lo_hi[6 * l];
+        const int r = ratio[l];
+        if (r < 1) return set_err(BSN_ERR_ARG, "ratio %d", r);
+        const int n[3] = {P.nx, P.ny, P.nz}, nc[3] = {C.nx, C.ny, C.nz};
+        if (r > 4) return set_err(BSN_ERR_ARG, "ratio %d > 4 not supported", r);
+        for (int a = 0; a < 3; ++a) {
+            const int64_t lo = b[2 * a], hi = b[2 * a + 1];
+            if (lo < 1 || hi > n[a] || hi < lo) return set_err(BSN_ERR_ARG, "level %d box axis %d", l + 1, a);
+            if ((hi - lo + 1) * r != nc[a]) return set_err(BSN_ERR_ARG, "level %d size mismatch axis %d", l + 1, a);
+            // every 5-point prolongation stencil (fmr3d.jl:165-198) must stay
+            // inside the parent's padded array: check the first and last fine
+            // index, between which the coarse centre is monotone
+            const int first = coarse_centre(1, (int)lo, r);
+            const int last = coarse_centre(nc[a] + 2 * BSN_NG, (int)lo, r);
+            if (BSN_NG + first - 3 < 0 || BSN_NG + last + 1 > n[a] + 2 * BSN_NG - 1)
+                return set_err(BSN_ERR_ARG, "level %d box too close to the parent edge (axis %d)", l + 1, a);
+        }
+    }
+    bsn_fmr* m = new bsn_fmr();
+    m->ctx = ctx;
+    m->nlev = nlevels;
+    m->lev.assign(levels, levels + nlevels);
+    m->alpha.assign(nlevels, 0.0);
+    auto fail = [&](int code) {
+        bsn_fmr_destroy(m);
+        return code;
+    };
+    for (int l = 0; l < nlevels; ++l) {
+        Plan* P = nullptr;
+        int r = plan_new(ctx, {levels[l]}, &P);
+        if (r != BSN_OK) return fail(r);
+        m->plans.push_back(P);
+    }
+    std::vector<GridDev> all;
+    for (auto* g : m->lev) all.push_back(g->h);
+    if (upload_vec(&m->d_all, all) != BSN_OK) return fail(BSN_ERR_HIP);
+    for (int l = 0; l + 1 < nlevels; ++l) {
+        FmrDev f{};
+        f.parent = l;
+        f.child = l + 1;
+        f.ratio = ratio[l];
+        for (int a = 0; a < 3; ++a) { f.lo[a] = (int)lo_hi[6 * l + 2 * a]; f.hi[a] = (int)lo_hi[6 * l + 2 * a + 1]; }
+        // src/fmr3d.jl:32-46
+        const double nodes[5] = {-2.0, -1.0, 0.0, 1.0, 2.0};
+        for (int s = 0; s < f.ratio && s < 4; ++s) {
+            const double t = ((double)s + 0.5 - (double)f.ratio / 2.0) / (double)f.ratio;
+            for (int j = 0; j < 5; ++j) {
+                double wj = 1.0;
+                for (int mm = 0; mm < 5; ++mm)
+                    if (mm != j) wj *= (t - nodes[mm]) / (nodes[j] - nodes[mm]);
+                f.w[s * 5 + j] = wj;
+            }
+        }
+        if (f.ratio > 4) return fail(set_err(BSN_ERR_ARG, "ratio > 4 not supported"));
+        const int nic = f.hi[0] - f.lo[0] + 1, njc = f.hi[1] - f.lo[1] + 1, nkc = f.hi[2] - f.lo[2] + 1;
+        const int64_t face[6] = {(int64_t)njc * nkc, (int64_t)njc * nkc, (int64_t)nic * nkc,
+                                 (int64_t)nic * nkc, (int64_t)nic * njc, (int64_t)nic * njc};
+        f.froff[0] = 0;
+        for (int k = 0; k < 6; ++k) f.froff[k + 1] = f.froff[k] + 5 * face[k];
+        if (hipMalloc((void**)&f.frc, f.froff[6] * 8) != hipSuccess ||
+            hipMalloc((void**)&f.frf, f.froff[6] * 8) != hipSuccess)
+            return fail(set_err(BSN_ERR_HIP, "flux register allocation"));
+        f.Ucn = levels[l]->h.Un;
+        f.Ucnp1 = levels[l]->h.U;
+        m->hf.push_back(f);
+    }
+    if (!m->hf.empty() && upload_vec(&m->d_f, m->hf) != BSN_OK) return fail(BSN_ERR_HIP);
+    *out = m;
+    return BSN_OK;
+}
+
+extern "C" int bsn_fmr_destroy(bsn_fmr* m) {
+    if (!m) return BSN_OK;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    graph_drop(m->gexec, m->graph);
+    for (auto* P : m->plans) plan_free(P);
+    for (auto& f : m->hf) { (void)hipFree(f.frc); (void)hipFree(f.frf); }
+    (void)hipFree(m->d_f);
+    (void)hipFree(m->d_all);
+    delete m;
+    return BSN_OK;
+}
+
+static int fmr_refresh(bsn_fmr* m) {
+    std::vector<GridDev> all;
+    for (auto* g : m->lev) all.push_back(g->h);
+    HIPCHK(hipMemcpy(m->d_all, all.data(), all.size() * sizeof(GridDev), hipMemcpyHostToDevice));
+    for (auto* P : m->plans) CHK(plan_refresh(*P));
+    return BSN_OK;
+}
+
+static int64_t ghost_total(const GridDev& d) {
+    return 2 * BSN_NG * ((int64_t)d.Ny * d.Nz + (int64_t)d.nx * d.Nz + (int64_t)d.nx * d.ny);
+}
+
+static int fmr_prolong(bsn_fmr* m, int l, int all_cells, bool interp) {
+    const FmrDev& f = m->hf[l - 1];
+    const GridDev& C = m->lev[l]->h;
+    const GridDev& Pg = m->lev[l - 1]->h;
+    const int64_t total = all_cells ? (int64_t)(C.nx + 2 * BSN_NG) * C.Ny * C.Nz : ghost_total(C);
+    HIPCHK(launch_prolong(m->ctx->L(), m->d_all, m->d_f + (l - 1), total,
+                          interp ? Pg.Un : Pg.U, interp ? Pg.U : nullptr, m->alpha[l], all_cells));
+    return BSN_OK;
+}
+
+static const double RK_W[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};   // fmr3d.jl:496
+
+// Advance level l by its dt (already in its scal): own RK3 with flux-register
+// accumulation, then the child's sub-cycles, restriction and correction.
+static int fmr_advance(bsn_fmr* m, int l) {
+    Plan& P = *m->plans[l];
+    const LaunchCtx L = m->ctx->L();
+    const bool child = l + 1 < m->nlev;
+    const GridDev& h = m->lev[l]->h;
+    if (child) HIPCHK(launch_fr_zero(L, m->d_f + l, m->hf[l].froff[6]));
+    if (l > 0) HIPCHK(launch_copy_state(L, h, h.U, h.Un));
+    for (int s = 1; s <= 3; ++s) {
+        if (l == 0) {
+            if (s == 1) { CHK(plan_ghost_fill(P, 1)); CHK(plan_floors(P, 1)); }
+            else CHK(plan_ghost_fill(P, 0));
+        } else {
+            CHK(fmr_prolong(m, l, 0, true));
+            CHK(plan_floors(P, 0));
+        }
+        CHK(plan_fluxes(P));
+        if (child) HIPCHK(launch_fr_coarse(L, P.prec, m->d_all, m->d_f + l, m->hf[l].froff[6] / 5, RK_W[s - 1]));
+        if (l > 0) HIPCHK(launch_fr_fine(L, P.prec, m->d_all, m->d_f + (l - 1), m->hf[l - 1].froff[6] / 5, RK_W[s - 1]));
+        CHK(plan_update(P, s, 0));
+    }
+    CHK(plan_ghost_combine(P, 3));
+    if (child) {
+        const FmrDev& f = m->hf[l];
+        HIPCHK(launch_child_dt(L, m->d_all, l, l + 1, f.ratio));
+        for (int ss = 0; ss < f.ratio; ++ss) {
+            m->alpha[l + 1] = (double)ss / (double)f.ratio;
+            CHK(fmr_advance(m, l + 1));
+        }
+        const int64_t ncells = (int64_t)(f.hi[0] - f.lo[0] + 1) * (f.hi[1] - f.lo[1] + 1) * (f.hi[2] - f.lo[2] + 1);
+        HIPCHK(launch_restrict(L, m->d_all, m->d_f + l, ncells));
+        HIPCHK(launch_bc_correct(L, m->d_all, m->d_f + l, f.froff[6] / 5));
+        CHK(plan_floors(P, 0));
+    }
+    return BSN_OK;
+}
+
+extern "C" int bsn_fmr_prolong_all(bsn_fmr* m, int level) {
+    if (!m || level < 1 || level >= m->nlev) return set_err(BSN_ERR_ARG, "bad level");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    CHK(fmr_refresh(m));
+    CHK(fmr_prolong(m, level, 1, false));
+    return sync_ctx(m->ctx);
+}
+
+extern "C" int bsn_fmr_prolong_fine_ghosts(bsn_fmr* m, int level) {
+    if (!m || level < 1 || level >= m->nlev) return set_err(BSN_ERR_ARG, "bad level");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    CHK(fmr_refresh(m));
+    CHK(fmr_prolong(m, level, 0, false));
+    return sync_ctx(m->ctx);
+}
+
+extern "C" int bsn_fmr_restrict(bsn_fmr* m, int level) {
+    if (!m || level < 1 || level >= m->nlev) return set_err(BSN_ERR_ARG, "bad level");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    const FmrDev& f = m->hf[level - 1];
+    const int64_t ncells = (int64_t)(f.hi[0] - f.lo[0] + 1) * (f.hi[1] - f.lo[1] + 1) * (f.hi[2] - f.lo[2] + 1);
+    HIPCHK(launch_restrict(m->ctx->L(), m->d_all, m->d_f + (level - 1), ncells));
+    return sync_ctx(m->ctx);
+}
+
+extern "C" int bsn_fmr3d_step(bsn_fmr* m, double dt_c) {
+    if (!m) return set_err(BSN_ERR_ARG, "NULL fmr");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    CHK(fmr_refresh(m));
+    HIPCHK(launch_set_scal(m->ctx->L(), m->lev[0]->h.scal, SCAL_DT, dt_c));
+    CHK(fmr_advance(m, 0));
+    return sync_ctx(m->ctx);
+}
+
+// Level CFL dts folded into the root: dt_0 = min(dt_0, r0 dt_1, r0 r1 dt_2, ...)
+static int fmr_cfl(bsn_fmr* m, double cfl, double t_end) {
+    const LaunchCtx L = m->ctx->L();
+    double factor = 1.0;
+    for (int l = 0; l < m->nlev; ++l) {
+        Plan& P = *m->plans[l];
+        CHK(plan_cfl(P, cfl, l == 0 ? t_end : 0.0));
+        HIPCHK(launch_dt(L, P.d_grids, 1));
+    }
+    for (int l = 1; l < m->nlev; ++l) {
+        factor *= (double)m->hf[l - 1].ratio;
+        HIPCHK(launch_fold_dt(L, m->d_all, 0, l, factor));
+    }
+    return BSN_OK;
+}
+
+extern "C" int bsn_cfl_dt_fmr3d(bsn_fmr* m, double cfl, double* dt_out) {
+    if (!m || !dt_out) return set_err(BSN_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    CHK(fmr_refresh(m));
+    CHK(fmr_cfl(m, cfl, 0.0));
+    HIPCHK(hipMemcpyAsync(dt_out, &m->lev[0]->h.scal[SCAL_DT], 8, hipMemcpyDeviceToHost, m->ctx->stream));
+    return sync_ctx(m->ctx);
+}
+
+static int fmr_device_step(bsn_fmr* m, double cfl, double t_end) {
+    CHK(fmr_cfl(m, cfl, t_end));
+    CHK(fmr_advance(m, 0));
+    HIPCHK(launch_advance_t(m->ctx->L(), m->d_all, 1));
+    return BSN_OK;
+}
+
+extern "C" int bsn_fmr_run(bsn_fmr* m, double cfl, double t_end, int nsteps, int use_graph) {
+    if (!m || nsteps < 0) return set_err(BSN_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    CHK(fmr_refresh(m));
+    const LaunchCtx L = m->ctx->L();
+    HIPCHK(launch_set_scal(L, m->lev[0]->h.scal, SCAL_T, 0.0));
+    if (!use_graph) {
+        for (int s = 0; s < nsteps; ++s) CHK(fmr_device_step(m, cfl, t_end));
+        return sync_ctx(m->ctx);
+    }
+    graph_drop(m->gexec, m->graph);
+    HIPCHK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+    int r = fmr_device_step(m, cfl, t_end);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(L.stream, &gr);
+    if (r != BSN_OK) { if (gr) (void)hipGraphDestroy(gr); return r; }
+    if (e != hipSuccess) return set_err(BSN_ERR_HIP, "capture: %s", hipGetErrorString(e));
+    m->graph = gr;
+    HIPCHK(hipGraphInstantiate(&m->gexec, m->graph, nullptr, nullptr, 0));
+    for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(m->gexec, L.stream));
+    return sync_ctx(m->ctx);
+}
+
+extern "C" int bsn_fmr_time(bsn_fmr* m, double* t, double* dt) {
+    if (!m) return set_err(BSN_ERR_ARG, "NULL fmr");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    double s[SCAL_COUNT];
+    HIPCHK(hipMemcpy(s, m->lev[0]->h.scal, sizeof s, hipMemcpyDeviceToHost));
+    if (t) *t = s[SCAL_T];
+    if (dt) *dt = s[SCAL_DT];
+    return BSN_OK;
+}

@@ -1,0 +1,1133 @@
+// bsn_kernels.hip -- CDNA4 kernels of the Euler right-hand side, the SSP-RK3
+// stage update, the CFL reduction, ghost fills and FMR transfer operators.
+//
+// Kernel map (reference function -> kernel):
+//   fill_ghost_3d_outflow!/periodic!  euler3d.jl:33-62        k_ghost_fill
+//   _apply_floors_kernel!             gpu_kernels.jl:26-46    k_floors (and inside k_update)
+//   _weno_fluxes_{x,y,z}_kernel!      gpu_kernels.jl:51-154   k_flux<DIR>
+//   _flux_divergence_kernel!          gpu_kernels.jl:159-169  k_update (fused)
+//   _sg_gradient_kernel!              gpu_kernels.jl:291-319  k_update (fused), k_sg_force
+//   RK3 stage combines                euler3d.jl:244,250,256  k_update (fused), k_ghost_combine
+//   _cfl_speeds_kernel! + maximum     gpu_kernels.jl:174-191  k_update (stage 3), k_cfl
+//   _prolong_fine_ghosts!/_prolong_all!  fmr3d.jl:206-255     k_prolong
+//   _restrict!                        fmr3d.jl:266-285        k_restrict
+//   _accumulate_fr_coarse!/_fine!     fmr3d.jl:292-369        k_fr_coarse, k_fr_fine
+//   _bc_correct!                      fmr3d.jl:379-422        k_bc_correct
+//
+// All work is described by device-resident tile lists (struct Tile) or by
+// per-grid prefix offsets, so one launch sweeps any number of grids of any
+// shape.  Tile-list kernels are persistent: a workgroup takes a contiguous
+// chunk of the list, which keeps neighbouring tiles (shared halo cells) on one
+// CU and one XCD L2.
+#include <hip/hip_runtime.h>
+#include "bsn_internal.hpp"
+#include "bsn_physics.hpp"
+#include "bsn_launch.hpp"
+
+namespace bsn {
+
+#define NG BSN_NG
+
+__device__ __forceinline__ int64_t uidx(const GridDev& G, int i, int j, int k) {
+    return ((int64_t)k * G.Ny + j) * G.pitch + BSN_XOFF + i;
+}
+
+__device__ __forceinline__ void chunk(int ntiles, int& t0, int& t1) {
+    t0 = (int)(((int64_t)blockIdx.x * ntiles) / gridDim.x);
+    t1 = (int)(((int64_t)(blockIdx.x + 1) * ntiles) / gridDim.x);
+}
+
+__device__ __forceinline__ int find_grid(const int64_t* __restrict__ off, int ngrids, int64_t e) {
+    int lo = 0, hi = ngrids - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Ghost cells.  The set of ghost cells of a grid is split into three disjoint
+// slabs: x-slabs (ghost i, all j, k), y-slabs (active i, ghost j, all k) and
+// z-slabs (active i, j, ghost k) -- the same partition as the reference's
+// fill order (euler3d.jl:36-43, fmr3d.jl:228-254).
+
+__device__ __forceinline__ int64_t ghost_count(const GridDev& G) {
+    return 2 * NG * ((int64_t)G.Ny * G.Nz + (int64_t)G.nx * G.Nz + (int64_t)G.nx * G.ny);
+}
+
+__device__ __forceinline__ void ghost_decode(const GridDev& G, int64_t t, int& i, int& j, int& k) {
+    const int64_t A = 2 * NG * (int64_t)G.Ny * G.Nz;
+    const int64_t B = 2 * NG * (int64_t)G.nx * G.Nz;
+    if (t < A) {
+        int ig = (int)(t % (2 * NG)); t /= 2 * NG;
+        j = (int)(t % G.Ny); k = (int)(t / G.Ny);
+        i = ig < NG ? ig : G.nx + ig;
+    } else if (t < A + B) {
+        t -= A;
+        i = NG + (int)(t % G.nx); t /= G.nx;
+        int jg = (int)(t % (2 * NG)); k = (int)(t / (2 * NG));
+        j = jg < NG ? jg : G.ny + jg;
+    } else {
+        t -= A + B;
+        i = NG + (int)(t % G.nx); t /= G.nx;
+        j = NG + (int)(t % G.ny);
+        int kg = (int)(t / G.ny);
+        k = kg < NG ? kg : G.nz + kg;
+    }
+}
+
+__device__ __forceinline__ int bc_src(int a, int n, bool periodic) {
+    if (periodic) return a < NG ? a + n : (a >= NG + n ? a - n : a);
+    return a < NG ? NG : (a >= NG + n ? NG + n - 1 : a);
+}
+
+// Ghost fill by boundary condition.  A ghost cell takes the value of the
+// active cell its index maps to per axis; this equals the reference's
+// x -> y -> z sequence of slab copies (each slab copy only moves values that
+// an earlier copy placed).  With copy_un, the raw value of every ghost cell
+// is first saved to Un (the RK3 base state is a copy of the input).
+__global__ void k_ghost_fill(const GridDev* __restrict__ grids, const int64_t* __restrict__ off,
+                             int ngrids, int copy_un) {
+    const int64_t total = off[ngrids];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int g = find_grid(off, ngrids, e);
+        const GridDev& G = grids[g];
+        int i, j, k;
+        ghost_decode(G, e - off[g], i, j, k);
+        const int si = bc_src(i, G.nx, G.bc & 1), sj = bc_src(j, G.ny, G.bc & 2),
+                  sk = bc_src(k, G.nz, G.bc & 4);
+        const int64_t d = uidx(G, i, j, k), s = uidx(G, si, sj, sk);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            if (copy_un) G.Un[q * G.fs + d] = G.U[q * G.fs + d];
+            G.U[q * G.fs + d] = G.U[q * G.fs + s];
+        }
+    }
+}
+
+// Stage-3 combination on ghost cells, where the reference's dU is zero
+// (euler3d.jl:138 fills dU with zeros; the combine runs over the whole array).
+__global__ void k_ghost_combine(const GridDev* __restrict__ grids, const int64_t* __restrict__ off,
+                                int ngrids, int stage) {
+    const int64_t total = off[ngrids];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int g = find_grid(off, ngrids, e);
+        const GridDev& G = grids[g];
+        int i, j, k;
+        ghost_decode(G, e - off[g], i, j, k);
+        const int64_t d = uidx(G, i, j, k);
+        const double dt = G.scal[SCAL_DT];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const double un = G.Un[q * G.fs + d], u = G.U[q * G.fs + d];
+            double v;
+            if (stage == 1)      v = un + dt * 0.0;
+            else if (stage == 2) v = 0.75 * un + 0.25 * u + (0.25 * dt) * 0.0;
+            else                 v = (1.0 / 3.0) * un + (2.0 / 3.0) * u + ((2.0 / 3.0) * dt) * 0.0;
+            G.U[q * G.fs + d] = v;
+        }
+    }
+}
+
+// src/gpu_kernels.jl:26-46 on the five values of one cell.
+__device__ __forceinline__ bool apply_floor(double (&u)[5], double rho_floor, double P_floor,
+                                            double g) {
+    const double r = u[0];
+    if (r < rho_floor) {
+        u[0] = rho_floor; u[1] = 0.0; u[2] = 0.0; u[3] = 0.0;
+        u[4] = P_floor / (g - 1.0);
+        return true;
+    }
+    const double KE = (0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3])) / r;
+    const double P = (g - 1.0) * (u[4] - KE);
+    if (P < P_floor) { u[4] = P_floor / (g - 1.0) + KE; return true; }
+    return false;
+}
+
+// Floors on active cells (apply_floors_3d!, euler3d.jl:73-80); with copy_un
+// the raw value is saved to Un first.
+__global__ void k_floors(const GridDev* __restrict__ grids, const int64_t* __restrict__ off,
+                         int ngrids, int copy_un) {
+    const int64_t total = off[ngrids];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int g = find_grid(off, ngrids, e);
+        const GridDev& G = grids[g];
+        int64_t t = e - off[g];
+        const int i = NG + (int)(t % G.nx); t /= G.nx;
+        const int j = NG + (int)(t % G.ny);
+        const int k = NG + (int)(t / G.ny);
+        const int64_t c = uidx(G, i, j, k);
+        double u[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + c];
+        if (copy_un) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) G.Un[q * G.fs + c] = u[q];
+        }
+        if (apply_floor(u, G.rho_floor, G.P_floor, G.gamma)) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) G.U[q * G.fs + c] = u[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Interface fluxes.  One workgroup stages a pencil tile of cells (five
+// conserved values reduced to rho, m, P) in LDS, six cells deep along the
+// sweep direction, then each thread computes one interface.
+//
+//   DIR 0 (x): tile (g, f0, j0, k): interfaces f0..f0+W-1 (f < nx) along x
+//              for rows j0..j0+H-1.  Lanes run along x.  A tile with
+//              a = -1 instead computes the last interface f = nx for the
+//              rows j0..j0+W*H-1, gathering from HBM.
+//   DIR 1 (y): tile (g, f0, i0, k): interfaces f0..f0+H-1 (f <= ny) along y
+//              for cells i0..i0+W-1.  Lanes run along x (coalesced).
+//   DIR 2 (z): tile (g, f0, i0, j): as DIR 1 along z.
+
+template <typename T> struct Prec;
+template <> struct Prec<double> {
+    template <int DIR>
+    __device__ static void flux(const double (&c)[4][6], const double (&p)[6], const GridDev& G,
+                                double F[5]) {
+        ex::interface_flux<DIR>(c, p, G.gamma, G.gfac, F);
+    }
+};
+template <> struct Prec<float> {
+    template <int DIR>
+    __device__ static void flux(const float (&c)[4][6], const float (&p)[6], const GridDev& G,
+                                float F[5]) {
+        sp::interface_flux<DIR>(c, p, (float)G.gamma, (float)G.gfac, (float)(1.0 / (G.gamma - 1.0)), F);
+    }
+};
+
+template <int DIR, typename T, int W, int H>
+__global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grids,
+                                                 const Tile* __restrict__ tiles, int ntiles) {
+    constexpr int NT = W * H;
+    constexpr int SA = DIR == 0 ? H : H + 5;     // LDS rows
+    constexpr int SB = DIR == 0 ? W + 5 : W;     // LDS row length
+    __shared__ T lds[5][SA][SB];
+    const int tid = threadIdx.x;
+    int t0, t1;
+    chunk(ntiles, t0, t1);
+    for (int t = t0; t < t1; ++t) {
+        const Tile tl = tiles[t];
+        const GridDev& G = grids[tl.g];
+        const int Nx = G.nx + 2 * NG;
+        T c[4][6], p[6];
+        bool valid;
+        int f, ci, cj;   // interface index along DIR, and the two cross indices (active, 0-based)
+        if (DIR == 0 && tl.a < 0) {
+            // last x interface f = nx of rows j0 + tid, gathered from HBM
+            f = G.nx; ci = tl.b + tid; cj = tl.c;
+            valid = ci < G.ny;
+            if (valid) {
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    const int64_t o = uidx(G, G.nx + m, NG + ci, NG + cj);
+                    double u[5];
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + o];
+                    const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) c[q][m] = (T)u[q];
+                    p[m] = (T)pr;
+                }
+            }
+        } else {
+            // stage cells into LDS
+            for (int e = tid; e < SA * SB; e += NT) {
+                const int a = e / SB, b = e - a * SB;
+                int pi, pj, pk;
+                bool ok;
+                if (DIR == 0) {
+                    pi = tl.a + b; pj = NG + tl.b + a; pk = NG + tl.c;
+                    ok = pi < Nx && tl.b + a < G.ny;
+                } else if (DIR == 1) {
+                    pi = NG + tl.b + b; pj = tl.a + a; pk = NG + tl.c;
+                    ok = tl.b + b < G.nx && pj < G.ny + 2 * NG;
+                } else {
+                    pi = NG + tl.b + b; pj = NG + tl.c; pk = tl.a + a;
+                    ok = tl.b + b < G.nx && pk < G.nz + 2 * NG;
+                }
+                if (ok) {
+                    const int64_t o = uidx(G, pi, pj, pk);
+                    double u[5];
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + o];
+                    const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) lds[q][a][b] = (T)u[q];
+                    lds[4][a][b] = (T)pr;
+                }
+            }
+            __syncthreads();
+            const int x = tid % W, h = tid / W;
+            if (DIR == 0) {
+                f = tl.a + x; ci = tl.b + h; cj = tl.c;
+                valid = f < G.nx && ci < G.ny;
+            } else {
+                f = tl.a + h; ci = tl.b + x; cj = tl.c;
+                valid = f <= (DIR == 1 ? G.ny : G.nz) && ci < G.nx;
+            }
+            if (valid) {
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        c[q][m] = DIR == 0 ? lds[q][h][x + m] : lds[q][h + m][x];
+                    p[m] = DIR == 0 ? lds[4][h][x + m] : lds[4][h + m][x];
+                }
+            }
+        }
+        if (valid) {
+            T F[5];
+            Prec<T>::template flux<DIR>(c, p, G, F);
+            T* Fp = (T*)G.F[DIR];
+            int64_t o;
+            if (DIR == 0)      o = ((int64_t)cj * G.ny + ci) * G.fp[0] + f;
+            else if (DIR == 1) o = ((int64_t)cj * (G.ny + 1) + f) * G.fp[1] + ci;
+            else               o = ((int64_t)f * G.ny + cj) * G.fp[2] + ci;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) Fp[q * G.ffs[DIR] + o] = F[q];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused flux divergence + phi source + SSP-RK3 stage combine + floors
+// (+ the CFL wave-speed maximum of the new state on stage 3).
+// Tile (g, i0, j0, k0): 64 x 4 threads over (i, j); each thread walks KZ
+// cells along z and carries the z-face flux.  stage 0 stores dU instead
+// (euler3d_rhs!).
+
+__device__ __forceinline__ uint64_t cfl_speed_bits(const double (&u)[5], const GridDev& G) {
+    // src/gpu_kernels.jl:180-189
+    const double g = G.gamma;
+    const double r = jl_max(u[0], 1e-30);
+    const double vx = u[1] / r, vy = u[2] / r, vz = u[3] / r;
+    const double KE = 0.5 * (vx * vx + vy * vy + vz * vz);
+    const double P = jl_max(((g - 1.0) * (u[4] / r - KE)) * r, 0.0);
+    const double cs = sqrt((g * P) / r);
+    const double s = jl_max(jl_max((fabs(vx) + cs) * G.idx, (fabs(vy) + cs) * G.idy),
+                            (fabs(vz) + cs) * G.idz);
+    return (uint64_t)__double_as_longlong(s);
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t w = (uint64_t)__shfl_xor((unsigned long long)v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void block_max_to(uint64_t v, unsigned long long* dst) {
+    __shared__ uint64_t red[16];
+    v = wave_max_u64(v);
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t m = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = red[w] > m ? red[w] : m;
+        if (m) atomicMax(dst, (unsigned long long)m);
+    }
+    __syncthreads();
+}
+
+template <typename T, int KZ>
+__global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grids,
+                                                const Tile* __restrict__ tiles, int ntiles,
+                                                int stage, int do_cfl, double* __restrict__ dU_out) {
+    int t0, t1;
+    chunk(ntiles, t0, t1);
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    for (int t = t0; t < t1; ++t) {
+        const Tile tl = tiles[t];
+        const GridDev& G = grids[tl.g];
+        const int i = tl.a + lx, j = tl.b + ly;
+        uint64_t smax = 0;
+        if (i < G.nx && j < G.ny) {
+            const T* Fx = (const T*)G.F[0];
+            const T* Fy = (const T*)G.F[1];
+            const T* Fz = (const T*)G.F[2];
+            const int kend = min(tl.c + KZ, G.nz);
+            const double dt = G.scal[SCAL_DT];
+            const double g = G.gamma;
+            double fz0[5];
+            {
+                const int64_t o = ((int64_t)tl.c * G.ny + j) * G.fp[2] + i;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) fz0[q] = (double)Fz[q * G.ffs[2] + o];
+            }
+            for (int k = tl.c; k < kend; ++k) {
+                const int64_t ox = ((int64_t)k * G.ny + j) * G.fp[0] + i;
+                const int64_t oy = ((int64_t)k * (G.ny + 1) + j) * G.fp[1] + i;
+                const int64_t oz = ((int64_t)(k + 1) * G.ny + j) * G.fp[2] + i;
+                double d[5], fz1[5];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    const double fx0 = (double)Fx[q * G.ffs[0] + ox];
+                    const double fx1 = (double)Fx[q * G.ffs[0] + ox + 1];
+                    const double fy0 = (double)Fy[q * G.ffs[1] + oy];
+                    const double fy1 = (double)Fy[q * G.ffs[1] + oy + G.fp[1]];
+                    fz1[q] = (double)Fz[q * G.ffs[2] + oz];
+                    // src/gpu_kernels.jl:165-167
+                    d[q] = (-(fx1 - fx0)) * G.idx - (fy1 - fy0) * G.idy - (fz1[q] - fz0[q]) * G.idz;
+                }
+                const int64_t c = uidx(G, NG + i, NG + j, NG + k);
+                if (stage == 0) {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) dU_out[q * G.fs + c] = d[q];
+                } else {
+                    double u[5], un[5];
+                    const bool need_u = stage > 1 || G.phi != nullptr;
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) {
+                        un[q] = G.Un[q * G.fs + c];
+                        u[q] = need_u ? G.U[q * G.fs + c] : 0.0;
+                    }
+                    if (G.phi) {
+                        // src/gpu_kernels.jl:297-317 (one-sided at active edges)
+                        const double* ph = G.phi;
+                        const int ip = min(i + 1, G.nx - 1), im = max(i - 1, 0);
+                        const int jp = min(j + 1, G.ny - 1), jm = max(j - 1, 0);
+                        const int kp = min(k + 1, G.nz - 1), km = max(k - 1, 0);
+                        const double i2x = 0.5 * G.idx, i2y = 0.5 * G.idy, i2z = 0.5 * G.idz;
+                        const double fxf = ip > im ? i2x : 2.0 * i2x;
+                        const double fyf = jp > jm ? i2y : 2.0 * i2y;
+                        const double fzf = kp > km ? i2z : 2.0 * i2z;
+                        const int64_t row = ((int64_t)k * G.ny + j) * G.nx;
+                        const double gx = (ph[row + ip] - ph[row + im]) * fxf;
+                        const double gy = (ph[((int64_t)k * G.ny + jp) * G.nx + i] -
+                                           ph[((int64_t)k * G.ny + jm) * G.nx + i]) * fyf;
+                        const double gz = (ph[((int64_t)kp * G.ny + j) * G.nx + i] -
+                                           ph[((int64_t)km * G.ny + j) * G.nx + i]) * fzf;
+                        const double r = u[0];
+                        const double vdotg = (u[1] * gx + u[2] * gy + u[3] * gz) / r;
+                        d[1] -= r * gx;
+                        d[2] -= r * gy;
+                        d[3] -= r * gz;
+                        d[4] -= r * vdotg;
+                    }
+                    double v[5];
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) {
+                        if (stage == 1)      v[q] = un[q] + dt * d[q];
+                        else if (stage == 2) v[q] = 0.75 * un[q] + 0.25 * u[q] + (0.25 * dt) * d[q];
+                        else                 v[q] = (1.0 / 3.0) * un[q] + (2.0 / 3.0) * u[q] + ((2.0 / 3.0) * dt) * d[q];
+                    }
+                    apply_floor(v, G.rho_floor, G.P_floor, g);
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) G.U[q * G.fs + c] = v[q];
+                    if (do_cfl) {
+                        const uint64_t b = cfl_speed_bits(v, G);
+                        smax = b > smax ? b : smax;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 5; ++q) fz0[q] = fz1[q];
+            }
+        }
+        if (do_cfl) block_max_to(smax, (unsigned long long*)&G.scal[SCAL_SMAX]);
+    }
+}
+
+// Stand-alone CFL maximum over active cells (cfl_dt_3d, euler3d.jl:157-169).
+template <int KZ>
+__global__ __launch_bounds__(256) void k_cfl(const GridDev* __restrict__ grids,
+                                             const Tile* __restrict__ tiles, int ntiles) {
+    int t0, t1;
+    chunk(ntiles, t0, t1);
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    for (int t = t0; t < t1; ++t) {
+        const Tile tl = tiles[t];
+        const GridDev& G = grids[tl.g];
+        const int i = tl.a + lx, j = tl.b + ly;
+        uint64_t smax = 0;
+        if (i < G.nx && j < G.ny) {
+            const int kend = min(tl.c + KZ, G.nz);
+            for (int k = tl.c; k < kend; ++k) {
+                const int64_t c = uidx(G, NG + i, NG + j, NG + k);
+                double u[5];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + c];
+                const uint64_t b = cfl_speed_bits(u, G);
+                smax = b > smax ? b : smax;
+            }
+        }
+        block_max_to(smax, (unsigned long long*)&G.scal[SCAL_SMAX]);
+    }
+}
+
+// dt = cfl / smax (Inf when smax is not > 0), optionally clipped to t_end - t;
+// the maximum is reset for the next accumulation.
+__global__ void k_dt(const GridDev* __restrict__ grids, int ngrids) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngrids) return;
+    double* s = grids[g].scal;
+    const uint64_t bits = *(const uint64_t*)&s[SCAL_SMAX];
+    const double smax = __longlong_as_double((long long)bits);
+    double dt = smax > 0.0 ? s[SCAL_CFL] / smax : __builtin_inf();
+    if (s[SCAL_TEND] > 0.0) dt = jl_min(dt, s[SCAL_TEND] - s[SCAL_T]);
+    s[SCAL_DT] = dt;
+    *(uint64_t*)&s[SCAL_SMAX] = 0;
+}
+
+__global__ void k_advance_t(const GridDev* __restrict__ grids, int ngrids) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngrids) return;
+    double* s = grids[g].scal;
+    s[SCAL_T] = s[SCAL_T] + s[SCAL_DT];
+}
+
+__global__ void k_set_scal(double* scal, int idx, double v) { scal[idx] = v; }
+
+__global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t v) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        p[e] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion between the reference's (5, Nx, Ny, Nz) column-major
+// array and the device layout.  dst/src select the device buffer (U, Un or
+// any buffer with U's layout).
+
+__global__ void k_aos_to_soa(GridDev G, const double* __restrict__ aos, double* __restrict__ dst) {
+    const int Nx = G.nx + 2 * NG;
+    const int64_t n = (int64_t)Nx * G.Ny * G.Nz;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % Nx);
+        const int64_t r = e / Nx;
+        const int j = (int)(r % G.Ny), k = (int)(r / G.Ny);
+        const int64_t c = uidx(G, i, j, k);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) dst[q * G.fs + c] = aos[5 * e + q];
+    }
+}
+
+__global__ void k_soa_to_aos(GridDev G, const double* __restrict__ src, double* __restrict__ aos) {
+    const int Nx = G.nx + 2 * NG;
+    const int64_t n = (int64_t)Nx * G.Ny * G.Nz;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % Nx);
+        const int64_t r = e / Nx;
+        const int j = (int)(r % G.Ny), k = (int)(r / G.Ny);
+        const int64_t c = uidx(G, i, j, k);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) aos[5 * e + q] = src[q * G.fs + c];
+    }
+}
+
+// Fluxes to the reference's F? arrays (zero outside the active-range
+// interfaces, as after euler3d.jl:197-202).  Shapes: Fx (5, Nx+1, Ny, Nz),
+// Fy (5, Nx, Ny+1, Nz), Fz (5, Nx, Ny, Nz+1).
+template <typename T>
+__global__ void k_flux_export(GridDev G, int dir, double* __restrict__ out) {
+    const int Nx = G.nx + 2 * NG;
+    const int Ex = Nx + (dir == 0), Ey = G.Ny + (dir == 1), Ez = G.Nz + (dir == 2);
+    const int64_t n = (int64_t)Ex * Ey * Ez;
+    const T* F = (const T*)G.F[dir];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(e % Ex);
+        const int64_t r = e / Ex;
+        const int b = (int)(r % Ey), c = (int)(r / Ey);
+        // reference 0-based index of interface f is NG - 1 + f along dir
+        int f = -1, ci = -1, cj = -1;
+        bool in;
+        if (dir == 0) { f = a - (NG - 1); ci = b - NG; cj = c - NG;
+                        in = f >= 0 && f <= G.nx && ci >= 0 && ci < G.ny && cj >= 0 && cj < G.nz; }
+        else if (dir == 1) { f = b - (NG - 1); ci = a - NG; cj = c - NG;
+                        in = f >= 0 && f <= G.ny && ci >= 0 && ci < G.nx && cj >= 0 && cj < G.nz; }
+        else { f = c - (NG - 1); ci = a - NG; cj = b - NG;
+               in = f >= 0 && f <= G.nz && ci >= 0 && ci < G.nx && cj >= 0 && cj < G.ny; }
+        int64_t o = 0;
+        if (in) {
+            if (dir == 0)      o = ((int64_t)cj * G.ny + ci) * G.fp[0] + f;
+            else if (dir == 1) o = ((int64_t)cj * (G.ny + 1) + f) * G.fp[1] + ci;
+            else               o = ((int64_t)f * G.ny + cj) * G.fp[2] + ci;
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) out[5 * e + q] = in ? (double)F[q * G.ffs[dir] + o] : 0.0;
+    }
+}
+
+// Phi source on a dU buffer with U's layout (_apply_sg_force_level!,
+// fmr_gravity.jl:146-156 -> gpu_kernels.jl:291-319).
+__global__ void k_sg_force(GridDev G, double* __restrict__ dU) {
+    const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % G.nx);
+        const int64_t r = e / G.nx;
+        const int j = (int)(r % G.ny), k = (int)(r / G.ny);
+        const int64_t c = uidx(G, NG + i, NG + j, NG + k);
+        const double* ph = G.phi;
+        const int ip = min(i + 1, G.nx - 1), im = max(i - 1, 0);
+        const int jp = min(j + 1, G.ny - 1), jm = max(j - 1, 0);
+        const int kp = min(k + 1, G.nz - 1), km = max(k - 1, 0);
+        const double i2x = 0.5 * G.idx, i2y = 0.5 * G.idy, i2z = 0.5 * G.idz;
+        const double fxf = ip > im ? i2x : 2.0 * i2x;
+        const double fyf = jp > jm ? i2y : 2.0 * i2y;
+        const double fzf = kp > km ? i2z : 2.0 * i2z;
+        const double gx = (ph[((int64_t)k * G.ny + j) * G.nx + ip] - ph[((int64_t)k * G.ny + j) * G.nx + im]) * fxf;
+        const double gy = (ph[((int64_t)k * G.ny + jp) * G.nx + i] - ph[((int64_t)k * G.ny + jm) * G.nx + i]) * fyf;
+        const double gz = (ph[((int64_t)kp * G.ny + j) * G.nx + i] - ph[((int64_t)km * G.ny + j) * G.nx + i]) * fzf;
+        const double rr = G.U[c], mx = G.U[G.fs + c], my = G.U[2 * G.fs + c], mz = G.U[3 * G.fs + c];
+        const double vdotg = (mx * gx + my * gy + mz * gz) / rr;
+        dU[G.fs + c] -= rr * gx;
+        dU[2 * G.fs + c] -= rr * gy;
+        dU[3 * G.fs + c] -= rr * gz;
+        dU[4 * G.fs + c] -= rr * vdotg;
+    }
+}
+
+// Whole-array RK3 combine with a caller-supplied dU in U's layout
+// (rk3.jl:24,28,32 / euler3d.jl:244,250,256), over every padded cell.
+__global__ void k_combine_all(GridDev G, int stage, const double* __restrict__ dU, double dt) {
+    const int Nx = G.nx + 2 * NG;
+    const int64_t n = (int64_t)Nx * G.Ny * G.Nz;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % Nx);
+        const int64_t r = e / Nx;
+        const int j = (int)(r % G.Ny), k = (int)(r / G.Ny);
+        const int64_t c = uidx(G, i, j, k);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const double un = G.Un[q * G.fs + c], u = G.U[q * G.fs + c], d = dU[q * G.fs + c];
+            double v;
+            if (stage == 1)      v = un + dt * d;
+            else if (stage == 2) v = 0.75 * un + 0.25 * u + (0.25 * dt) * d;
+            else                 v = (1.0 / 3.0) * un + (2.0 / 3.0) * u + ((2.0 / 3.0) * dt) * d;
+            G.U[q * G.fs + c] = v;
+        }
+    }
+}
+
+// Copy every padded cell of src into dst (both with U's layout).
+__global__ void k_copy_state(GridDev G, const double* __restrict__ src, double* __restrict__ dst) {
+    const int64_t n = 5 * G.fs;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = src[e];
+}
+
+// Conservation totals: per-block partial sums over active cells, then a
+// fixed-order final sum (deterministic for a fixed launch shape).
+__global__ void k_totals_partial(GridDev G, double* __restrict__ part) {
+    __shared__ double red[5][256];
+    const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % G.nx);
+        const int64_t r = e / G.nx;
+        const int j = (int)(r % G.ny), k = (int)(r / G.ny);
+        const int64_t c = uidx(G, NG + i, NG + j, NG + k);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) acc[q] += G.U[q * G.fs + c];
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) part[5 * blockIdx.x + q] = red[q][0];
+}
+
+__global__ void k_totals_final(const double* __restrict__ part, int nblocks, double dV,
+                               double* __restrict__ out) {
+    const int q = threadIdx.x;
+    if (q >= 5) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[5 * b + q];
+    out[q] = s * dV;
+}
+
+// ---------------------------------------------------------------------------
+// FMR transfer operators (src/fmr3d.jl).  Coarse index arithmetic follows
+// _coarse_stencil (fmr3d.jl:165-170) in FP64 with ties-to-even rounding.
+
+__device__ __forceinline__ void coarse_stencil(int ig_f1, int c_lo, int ratio, int& ic0, int& sx) {
+    // ig_f1 is the reference's 1-based fine index
+    const double xi = ((double)c_lo - 0.5) + ((double)(ig_f1 - NG) - 0.5) / (double)ratio;
+    const double r = rint(xi);
+    ic0 = (int)r;
+    int s = (int)floor(((xi - r) + 0.5) * (double)ratio);
+    sx = s < 0 ? 0 : (s > ratio - 1 ? ratio - 1 : s);
+}
+
+// Prolongation of fine cells from the parent state interpolated in time,
+// Uc(alpha) = (1 - alpha) Ucn + alpha Ucnp1 (fmr3d.jl:551), evaluated per
+// coarse value on the fly.  all_cells = 1 fills every fine cell
+// (_prolong_all!), else only ghost cells in the reference's slab partition.
+__global__ void k_prolong(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr,
+                          const double* __restrict__ Uca, const double* __restrict__ Ucb,
+                          double alpha, int all_cells) {
+    const FmrDev& M = *fmr;
+    const GridDev& C = grids[M.parent];
+    const GridDev& Fg = grids[M.child];
+    const int Nfx = Fg.nx + 2 * NG;
+    const int64_t total = all_cells ? (int64_t)Nfx * Fg.Ny * Fg.Nz : ghost_count(Fg);
+    const double oma = 1.0 - alpha;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        if (all_cells) {
+            i = (int)(e % Nfx);
+            const int64_t r = e / Nfx;
+            j = (int)(r % Fg.Ny); k = (int)(r / Fg.Ny);
+        } else {
+            ghost_decode(Fg, e, i, j, k);
+        }
+        int ic0, jc0, kc0, si, sj, sk;
+        coarse_stencil(i + 1, M.lo[0], M.ratio, ic0, si);
+        coarse_stencil(j + 1, M.lo[1], M.ratio, jc0, sj);
+        coarse_stencil(k + 1, M.lo[2], M.ratio, kc0, sk);
+        const int64_t d = uidx(Fg, i, j, k);
+        for (int q = 0; q < 5; ++q) {
+            const double* A = Uca + q * C.fs;
+            const double* B = Ucb ? Ucb + q * C.fs : nullptr;
+            double val = 0.0;
+            for (int di = -2; di <= 2; ++di) {
+                const int ig = NG + ic0 + di - 1;
+                const double wi = M.w[si * 5 + di + 2];
+                for (int dj = -2; dj <= 2; ++dj) {
+                    const int jg = NG + jc0 + dj - 1;
+                    const double wij = wi * M.w[sj * 5 + dj + 2];
+                    for (int dk = -2; dk <= 2; ++dk) {
+                        const int kg = NG + kc0 + dk - 1;
+                        const int64_t o = uidx(C, ig, jg, kg);
+                        const double uc = B ? oma * A[o] + alpha * B[o] : A[o];
+                        val += (wij * M.w[sk * 5 + dk + 2]) * uc;
+                    }
+                }
+            }
+            Fg.U[q * Fg.fs + d] = val;
+        }
+    }
+}
+
+// _restrict! (fmr3d.jl:266-285)
+__global__ void k_restrict(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr) {
+    const FmrDev& M = *fmr;
+    const GridDev& C = grids[M.parent];
+    const GridDev& Fg = grids[M.child];
+    const int r = M.ratio;
+    const int nic = M.hi[0] - M.lo[0] + 1, njc = M.hi[1] - M.lo[1] + 1, nkc = M.hi[2] - M.lo[2] + 1;
+    const int64_t total = (int64_t)nic * njc * nkc;
+    const double inv_r3 = 1.0 / (double)(r * r * r);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int m = (int)(e % nic);
+        const int64_t rr = e / nic;
+        const int n = (int)(rr % njc), p = (int)(rr / njc);
+        const int64_t dc = uidx(C, NG + M.lo[0] + m - 1, NG + M.lo[1] + n - 1, NG + M.lo[2] + p - 1);
+        for (int q = 0; q < 5; ++q) {
+            double val = 0.0;
+            for (int sx = 1; sx <= r; ++sx)
+                for (int sy = 1; sy <= r; ++sy)
+                    for (int sz = 1; sz <= r; ++sz)
+                        val += Fg.U[q * Fg.fs + uidx(Fg, NG + r * m + sx - 1, NG + r * n + sy - 1,
+                                                     NG + r * p + sz - 1)];
+            C.U[q * C.fs + dc] = val * inv_r3;
+        }
+    }
+}
+
+// Flux registers: face f in 0..5 = x_lo, x_hi, y_lo, y_hi, z_lo, z_hi, each
+// (5, n1, n2) column-major as FluxReg3D (fmr3d.jl:81-96).
+__device__ __forceinline__ void face_dims(const FmrDev& M, int face, int& n1, int& n2) {
+    const int nic = M.hi[0] - M.lo[0] + 1, njc = M.hi[1] - M.lo[1] + 1, nkc = M.hi[2] - M.lo[2] + 1;
+    if (face < 2)      { n1 = njc; n2 = nkc; }
+    else if (face < 4) { n1 = nic; n2 = nkc; }
+    else               { n1 = nic; n2 = njc; }
+}
+
+__global__ void k_fr_zero(const FmrDev* __restrict__ fmr) {
+    const FmrDev& M = *fmr;
+    const int64_t n = M.froff[6];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        M.frc[e] = 0.0;
+        M.frf[e] = 0.0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ double flux_at(const GridDev& G, int dir, int q, int f, int a, int b) {
+    // dir 0: (f, j=a, k=b); dir 1: (i=a, f, k=b); dir 2: (i=a, j=b, f)
+    const T* F = (const T*)G.F[dir];
+    int64_t o;
+    if (dir == 0)      o = ((int64_t)b * G.ny + a) * G.fp[0] + f;
+    else if (dir == 1) o = ((int64_t)b * (G.ny + 1) + f) * G.fp[1] + a;
+    else               o = ((int64_t)f * G.ny + b) * G.fp[2] + a;
+    return (double)F[q * G.ffs[dir] + o];
+}
+
+// _accumulate_fr_coarse! (fmr3d.jl:292-326); fac = weight * dt of the parent.
+template <typename T>
+__global__ void k_fr_coarse(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr,
+                            double weight) {
+    const FmrDev& M = *fmr;
+    const GridDev& C = grids[M.parent];
+    const double fac = weight * C.scal[SCAL_DT];
+    const int64_t n = M.froff[6] / 5;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int face = 0;
+        while (face < 5 && e >= M.froff[face + 1] / 5) ++face;
+        const int64_t loc = e - M.froff[face] / 5;
+        int n1, n2;
+        face_dims(M, face, n1, n2);
+        const int a = (int)(loc % n1), b = (int)(loc / n1);
+        const int hi = face & 1;
+        // reference interface NG + ci_lo - 1 (lo) or NG + ci_hi (hi) is
+        // device interface f = ci_lo - 1 or ci_hi (active-cell faces)
+        for (int q = 0; q < 5; ++q) {
+            double F;
+            if (face < 2)      F = flux_at<T>(C, 0, q, hi ? M.hi[0] : M.lo[0] - 1, M.lo[1] - 1 + a, M.lo[2] - 1 + b);
+            else if (face < 4) F = flux_at<T>(C, 1, q, hi ? M.hi[1] : M.lo[1] - 1, M.lo[0] - 1 + a, M.lo[2] - 1 + b);
+            else               F = flux_at<T>(C, 2, q, hi ? M.hi[2] : M.lo[2] - 1, M.lo[0] - 1 + a, M.lo[1] - 1 + b);
+            double* R = &M.frc[M.froff[face] + q + 5 * loc];
+            *R = *R + fac * F;
+        }
+    }
+}
+
+// _accumulate_fr_fine! (fmr3d.jl:331-369): each register element sums its
+// r^2 fine faces in the reference's loop order; fac = weight*dt_f/r^2.
+template <typename T>
+__global__ void k_fr_fine(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr,
+                          double weight) {
+    const FmrDev& M = *fmr;
+    const GridDev& Fg = grids[M.child];
+    const int r = M.ratio;
+    const double fac = (weight * Fg.scal[SCAL_DT]) / (double)(r * r);
+    const int64_t n = M.froff[6] / 5;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int face = 0;
+        while (face < 5 && e >= M.froff[face + 1] / 5) ++face;
+        const int64_t loc = e - M.froff[face] / 5;
+        int n1, n2;
+        face_dims(M, face, n1, n2);
+        const int a = (int)(loc % n1), b = (int)(loc / n1);
+        const int hi = face & 1;
+        for (int q = 0; q < 5; ++q) {
+            double* R = &M.frf[M.froff[face] + q + 5 * loc];
+            double acc = *R;
+            // outer loop over the second register index (kf or jf), inner
+            // over the first (jf or ia), as the reference
+            for (int ob = 0; ob < r; ++ob)
+                for (int oa = 0; oa < r; ++oa) {
+                    const int fa = a * r + oa, fb = b * r + ob;   // 0-based fine active
+                    double F;
+                    if (face < 2)      F = flux_at<T>(Fg, 0, q, hi ? Fg.nx : 0, fa, fb);
+                    else if (face < 4) F = flux_at<T>(Fg, 1, q, hi ? Fg.ny : 0, fa, fb);
+                    else               F = flux_at<T>(Fg, 2, q, hi ? Fg.nz : 0, fa, fb);
+                    acc = acc + fac * F;
+                }
+            *R = acc;
+        }
+    }
+}
+
+// _bc_correct! (fmr3d.jl:379-422): every corrected coarse cell receives one
+// face's correction, so the faces are applied in parallel.
+__global__ void k_bc_correct(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr) {
+    const FmrDev& M = *fmr;
+    const GridDev& C = grids[M.parent];
+    const int64_t n = M.froff[6] / 5;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int face = 0;
+        while (face < 5 && e >= M.froff[face + 1] / 5) ++face;
+        const int64_t loc = e - M.froff[face] / 5;
+        int n1, n2;
+        face_dims(M, face, n1, n2);
+        const int a = (int)(loc % n1), b = (int)(loc / n1);
+        const int hi = face & 1;
+        int ci, cj, ck;   // 1-based coarse active
+        double inv;
+        if (face < 2)      { ci = hi ? M.hi[0] + 1 : M.lo[0] - 1; cj = M.lo[1] + a; ck = M.lo[2] + b; inv = C.idx; }
+        else if (face < 4) { ci = M.lo[0] + a; cj = hi ? M.hi[1] + 1 : M.lo[1] - 1; ck = M.lo[2] + b; inv = C.idy; }
+        else               { ci = M.lo[0] + a; cj = M.lo[1] + b; ck = hi ? M.hi[2] + 1 : M.lo[2] - 1; inv = C.idz; }
+        const int64_t c = uidx(C, NG + ci - 1, NG + cj - 1, NG + ck - 1);
+        for (int q = 0; q < 5; ++q) {
+            const double rc = M.frc[M.froff[face] + q + 5 * loc];
+            const double rf = M.frf[M.froff[face] + q + 5 * loc];
+            const double delta = hi ? (rf - rc) * inv : (rc - rf) * inv;
+            C.U[q * C.fs + c] += delta;
+        }
+    }
+}
+
+// dt of a nested level: dt_child = dt_parent / ratio (fmr3d.jl:454).
+__global__ void k_child_dt(const GridDev* __restrict__ grids, int parent, int child, int ratio) {
+    grids[child].scal[SCAL_DT] = grids[parent].scal[SCAL_DT] / (double)ratio;
+}
+
+// dt_c = min(dt_c, r * dt_f) over a hierarchy (cfl_dt_fmr3d, fmr3d.jl:616-622):
+// level dts are computed first by k_dt on every grid, then folded into the root.
+__global__ void k_fold_dt(const GridDev* __restrict__ grids, int root, int child, double factor) {
+    double* s = grids[root].scal;
+    s[SCAL_DT] = jl_min(s[SCAL_DT], factor * grids[child].scal[SCAL_DT]);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+
+static int occupancy_grid(const void* fn, int threads, size_t lds, int ntiles, int cus) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds) != hipSuccess || per < 1)
+        per = 1;
+    int64_t g = (int64_t)per * cus;
+    if (g > ntiles) g = ntiles;
+    return (int)(g < 1 ? 1 : g);
+}
+
+template <int DIR, typename T, int W, int H>
+static hipError_t launch_flux_t(const LaunchCtx& L, const GridDev* grids, const Tile* tiles,
+                                int ntiles, int max_wg) {
+    if (ntiles <= 0) return hipSuccess;
+    const void* fn = (const void*)&k_flux<DIR, T, W, H>;
+    int g = occupancy_grid(fn, W * H, 0, ntiles, L.cus);
+    if (max_wg > 0 && g > max_wg) g = max_wg;
+    hipLaunchKernelGGL((k_flux<DIR, T, W, H>), dim3(g), dim3(W * H), 0, L.stream, grids, tiles, ntiles);
+    return hipGetLastError();
+}
+
+template <int DIR, typename T>
+static hipError_t launch_flux_shape(const LaunchCtx& L, int shape, const GridDev* grids,
+                                    const Tile* tiles, int ntiles, int max_wg) {
+    // shape ids: x sweep 0: 64x4, 1: 32x8, 2: 16x16;  y/z sweep 0: 64x4, 1: 64x8, 2: 32x8, 3: 64x2
+    if (DIR == 0) {
+        switch (shape) {
+            case 1: return launch_flux_t<DIR, T, 32, 8>(L, grids, tiles, ntiles, max_wg);
+            case 2: return launch_flux_t<DIR, T, 16, 16>(L, grids, tiles, ntiles, max_wg);
+            default: return launch_flux_t<DIR, T, 64, 4>(L, grids, tiles, ntiles, max_wg);
+        }
+    } else {
+        switch (shape) {
+            case 1: return launch_flux_t<DIR, T, 64, 8>(L, grids, tiles, ntiles, max_wg);
+            case 2: return launch_flux_t<DIR, T, 32, 8>(L, grids, tiles, ntiles, max_wg);
+            case 3: return launch_flux_t<DIR, T, 64, 2>(L, grids, tiles, ntiles, max_wg);
+            default: return launch_flux_t<DIR, T, 64, 4>(L, grids, tiles, ntiles, max_wg);
+        }
+    }
+}
+
+void flux_shape_dims(int dir, int shape, int& W, int& H) {
+    static const int xs[3][2] = {{64, 4}, {32, 8}, {16, 16}};
+    static const int ys[4][2] = {{64, 4}, {64, 8}, {32, 8}, {64, 2}};
+    if (dir == 0) { if (shape < 0 || shape > 2) shape = 0; W = xs[shape][0]; H = xs[shape][1]; }
+    else          { if (shape < 0 || shape > 3) shape = 0; W = ys[shape][0]; H = ys[shape][1]; }
+}
+
+int flux_shape_count(int dir) { return dir == 0 ? 3 : 4; }
+
+hipError_t launch_flux(const LaunchCtx& L, int dir, int prec, int shape, const GridDev* grids,
+                       const Tile* tiles, int ntiles, int max_wg) {
+    if (prec == 0) {
+        if (dir == 0) return launch_flux_shape<0, double>(L, shape, grids, tiles, ntiles, max_wg);
+        if (dir == 1) return launch_flux_shape<1, double>(L, shape, grids, tiles, ntiles, max_wg);
+        return launch_flux_shape<2, double>(L, shape, grids, tiles, ntiles, max_wg);
+    }
+    if (dir == 0) return launch_flux_shape<0, float>(L, shape, grids, tiles, ntiles, max_wg);
+    if (dir == 1) return launch_flux_shape<1, float>(L, shape, grids, tiles, ntiles, max_wg);
+    return launch_flux_shape<2, float>(L, shape, grids, tiles, ntiles, max_wg);
+}
+
+hipError_t launch_update(const LaunchCtx& L, int prec, const GridDev* grids, const Tile* tiles,
+                         int ntiles, int stage, int do_cfl, double* dU_out) {
+    if (ntiles <= 0) return hipSuccess;
+    if (prec == 0) {
+        const void* fn = (const void*)&k_update<double, UPDATE_KZ>;
+        const int g = occupancy_grid(fn, 256, 0, ntiles, L.cus);
+        hipLaunchKernelGGL((k_update<double, UPDATE_KZ>), dim3(g), dim3(256), 0, L.stream, grids,
+                           tiles, ntiles, stage, do_cfl, dU_out);
+    } else {
+        const void* fn = (const void*)&k_update<float, UPDATE_KZ>;
+        const int g = occupancy_grid(fn, 256, 0, ntiles, L.cus);
+        hipLaunchKernelGGL((k_update<float, UPDATE_KZ>), dim3(g), dim3(256), 0, L.stream, grids,
+                           tiles, ntiles, stage, do_cfl, dU_out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl(const LaunchCtx& L, const GridDev* grids, const Tile* tiles, int ntiles) {
+    if (ntiles <= 0) return hipSuccess;
+    const void* fn = (const void*)&k_cfl<UPDATE_KZ>;
+    const int g = occupancy_grid(fn, 256, 0, ntiles, L.cus);
+    hipLaunchKernelGGL((k_cfl<UPDATE_KZ>), dim3(g), dim3(256), 0, L.stream, grids, tiles, ntiles);
+    return hipGetLastError();
+}
+
+static int flat_blocks(int64_t n, int cus) {
+    int64_t b = (n + 255) / 256;
+    const int64_t cap = (int64_t)cus * 16;
+    if (b > cap) b = cap;
+    return (int)(b < 1 ? 1 : b);
+}
+
+hipError_t launch_ghost_fill(const LaunchCtx& L, const GridDev* grids, const int64_t* off,
+                             int ngrids, int64_t total, int copy_un) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ghost_fill, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids,
+                       off, ngrids, copy_un);
+    return hipGetLastError();
+}
+
+hipError_t launch_ghost_combine(const LaunchCtx& L, const GridDev* grids, const int64_t* off,
+                                int ngrids, int64_t total, int stage) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ghost_combine, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream,
+                       grids, off, ngrids, stage);
+    return hipGetLastError();
+}
+
+hipError_t launch_floors(const LaunchCtx& L, const GridDev* grids, const int64_t* off, int ngrids,
+                         int64_t total, int copy_un) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_floors, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids,
+                       off, ngrids, copy_un);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt(const LaunchCtx& L, const GridDev* grids, int ngrids) {
+    hipLaunchKernelGGL(k_dt, dim3((ngrids + 63) / 64), dim3(64), 0, L.stream, grids, ngrids);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance_t(const LaunchCtx& L, const GridDev* grids, int ngrids) {
+    hipLaunchKernelGGL(k_advance_t, dim3((ngrids + 63) / 64), dim3(64), 0, L.stream, grids, ngrids);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_scal(const LaunchCtx& L, double* scal, int idx, double v) {
+    hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, L.stream, scal, idx, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(const LaunchCtx& L, uint64_t* p, int64_t n, uint64_t v) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3(flat_blocks(n, L.cus)), dim3(256), 0, L.stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_aos_to_soa(const LaunchCtx& L, const GridDev& G, const double* aos, double* dst) {
+    const int64_t n = (int64_t)(G.nx + 2 * NG) * G.Ny * G.Nz;
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(flat_blocks(n, L.cus)), dim3(256), 0, L.stream, G, aos, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(const LaunchCtx& L, const GridDev& G, const double* src, double* aos) {
+    const int64_t n = (int64_t)(G.nx + 2 * NG) * G.Ny * G.Nz;
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(flat_blocks(n, L.cus)), dim3(256), 0, L.stream, G, src, aos);
+    return hipGetLastError();
+}
+
+hipError_t launch_flux_export(const LaunchCtx& L, const GridDev& G, int dir, double* out) {
+    const int64_t n = (int64_t)(G.nx + 2 * NG + (dir == 0)) * (G.Ny + (dir == 1)) * (G.Nz + (dir == 2));
+    if (G.prec == 0)
+        hipLaunchKernelGGL(k_flux_export<double>, dim3(flat_blocks(n, L.cus)), dim3(256), 0, L.stream, G, dir, out);
+    else
+        hipLaunchKernelGGL(k_flux_export<float>, dim3(flat_blocks(n, L.cus)), dim3(256), 0, L.stream, G, dir, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sg_force(const LaunchCtx& L, const GridDev& G, double* dU) {
+    const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    hipLaunchKernelGGL(k_sg_force, dim3(flat_blocks(n, L.cus)), dim3(256), 0, L.stream, G, dU);
+    return hipGetLastError();
+}
+
+hipError_t launch_combine_all(const LaunchCtx& L, const GridDev& G, int stage, const double* dU, double dt) {
+    const int64_t n = (int64_t)(G.nx + 2 * NG) * G.Ny * G.Nz;
+    hipLaunchKernelGGL(k_combine_all, dim3(flat_blocks(n, L.cus)), dim3(256), 0, L.stream, G, stage, dU, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_state(const LaunchCtx& L, const GridDev& G, const double* src, double* dst) {
+    hipLaunchKernelGGL(k_copy_state, dim3(flat_blocks(5 * G.fs, L.cus)), dim3(256), 0, L.stream, G, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_totals(const LaunchCtx& L, const GridDev& G, double* part, int nblocks, double dV,
+                         double* out) {
+    hipLaunchKernelGGL(k_totals_partial, dim3(nblocks), dim3(256), 0, L.stream, G, part);
+    hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(64), 0, L.stream, part, nblocks, dV, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_prolong(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total,
+                          const double* Uca, const double* Ucb, double alpha, int all_cells) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prolong, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids, fmr,
+                       Uca, Ucb, alpha, all_cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_restrict(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total) {
+    hipLaunchKernelGGL(k_restrict, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids, fmr);
+    return hipGetLastError();
+}
+
+hipError_t launch_fr_zero(const LaunchCtx& L, const FmrDev* fmr, int64_t total) {
+    hipLaunchKernelGGL(k_fr_zero, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, fmr);
+    return hipGetLastError();
+}
+
+hipError_t launch_fr_coarse(const LaunchCtx& L, int prec, const GridDev* grids, const FmrDev* fmr,
+                            int64_t nelem, double weight) {
+    if (prec == 0)
+        hipLaunchKernelGGL(k_fr_coarse<double>, dim3(flat_blocks(nelem, L.cus)), dim3(256), 0, L.stream, grids, fmr, weight);
+    else
+        hipLaunchKernelGGL(k_fr_coarse<float>, dim3(flat_blocks(nelem, L.cus)), dim3(256), 0, L.stream, grids, fmr, weight);
+    return hipGetLastError();
+}
+
+hipError_t launch_fr_fine(const LaunchCtx& L, int prec, const GridDev* grids, const FmrDev* fmr,
+                          int64_t nelem, double weight) {
+    if (prec == 0)
+        hipLaunchKernelGGL(k_fr_fine<double>, dim3(flat_blocks(nelem, L.cus)), dim3(256), 0, L.stream, grids, fmr, weight);
+    else
+        hipLaunchKernelGGL(k_fr_fine<float>, dim3(flat_blocks(nelem, L.cus)), dim3(256), 0, L.stream, grids, fmr, weight);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_correct(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t nelem) {
+    hipLaunchKernelGGL(k_bc_correct, dim3(flat_blocks(nelem, L.cus)), dim3(256), 0, L.stream, grids, fmr);
+    return hipGetLastError();
+}
+
+hipError_t launch_child_dt(const LaunchCtx& L, const GridDev* grids, int parent, int child, int ratio) {
+    hipLaunchKernelGGL(k_child_dt, dim3(1), dim3(1), 0, L.stream, grids, parent, child, ratio);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_dt(const LaunchCtx& L, const GridDev* grids, int root, int child, double factor) {
+    hipLaunchKernelGGL(k_fold_dt, dim3(1), dim3(1), 0, L.stream, grids, root, child, factor);
+    return hipGetLastError();
+}
+
+}  // namespace bsn

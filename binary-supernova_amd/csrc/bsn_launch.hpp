@@ -1,0 +1,54 @@
+// bsn_launch.hpp -- host-side launch helpers for the kernels in bsn_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "bsn_internal.hpp"
+
+#define UPDATE_KZ 8   // cells per thread along z in the update and CFL kernels
+
+namespace bsn {
+
+struct LaunchCtx {
+    hipStream_t stream;
+    int cus;          // compute units of the device
+};
+
+void flux_shape_dims(int dir, int shape, int& W, int& H);
+int flux_shape_count(int dir);
+
+hipError_t launch_flux(const LaunchCtx& L, int dir, int prec, int shape, const GridDev* grids,
+                       const Tile* tiles, int ntiles, int max_wg);
+hipError_t launch_update(const LaunchCtx& L, int prec, const GridDev* grids, const Tile* tiles,
+                         int ntiles, int stage, int do_cfl, double* dU_out);
+hipError_t launch_cfl(const LaunchCtx& L, const GridDev* grids, const Tile* tiles, int ntiles);
+hipError_t launch_ghost_fill(const LaunchCtx& L, const GridDev* grids, const int64_t* off,
+                             int ngrids, int64_t total, int copy_un);
+hipError_t launch_ghost_combine(const LaunchCtx& L, const GridDev* grids, const int64_t* off,
+                                int ngrids, int64_t total, int stage);
+hipError_t launch_floors(const LaunchCtx& L, const GridDev* grids, const int64_t* off, int ngrids,
+                         int64_t total, int copy_un);
+hipError_t launch_dt(const LaunchCtx& L, const GridDev* grids, int ngrids);
+hipError_t launch_advance_t(const LaunchCtx& L, const GridDev* grids, int ngrids);
+hipError_t launch_set_scal(const LaunchCtx& L, double* scal, int idx, double v);
+hipError_t launch_fill_u64(const LaunchCtx& L, uint64_t* p, int64_t n, uint64_t v);
+hipError_t launch_aos_to_soa(const LaunchCtx& L, const GridDev& G, const double* aos, double* dst);
+hipError_t launch_soa_to_aos(const LaunchCtx& L, const GridDev& G, const double* src, double* aos);
+hipError_t launch_flux_export(const LaunchCtx& L, const GridDev& G, int dir, double* out);
+hipError_t launch_sg_force(const LaunchCtx& L, const GridDev& G, double* dU);
+hipError_t launch_combine_all(const LaunchCtx& L, const GridDev& G, int stage, const double* dU,
+                              double dt);
+hipError_t launch_copy_state(const LaunchCtx& L, const GridDev& G, const double* src, double* dst);
+hipError_t launch_totals(const LaunchCtx& L, const GridDev& G, double* part, int nblocks, double dV,
+                         double* out);
+hipError_t launch_prolong(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total,
+                          const double* Uca, const double* Ucb, double alpha, int all_cells);
+hipError_t launch_restrict(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total);
+hipError_t launch_fr_zero(const LaunchCtx& L, const FmrDev* fmr, int64_t total);
+hipError_t launch_fr_coarse(const LaunchCtx& L, int prec, const GridDev* grids, const FmrDev* fmr,
+                            int64_t nelem, double weight);
+hipError_t launch_fr_fine(const LaunchCtx& L, int prec, const GridDev* grids, const FmrDev* fmr,
+                          int64_t nelem, double weight);
+hipError_t launch_bc_correct(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t nelem);
+hipError_t launch_child_dt(const LaunchCtx& L, const GridDev* grids, int parent, int child, int ratio);
+hipError_t launch_fold_dt(const LaunchCtx& L, const GridDev* grids, int root, int child, double factor);
+
+}  // namespace bsn

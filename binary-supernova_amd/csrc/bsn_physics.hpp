@@ -1,0 +1,403 @@
+// bsn_physics.hpp -- per-interface numerics of the Euler right-hand side on
+// CDNA4 (gfx950): WENO5-Z reconstruction, cell pressure, interface energy and
+// the HLLC flux.
+//
+// Two forms:
+//   * exact FP64 (namespace ex): every operation of src/weno5.jl:14-61,
+//     src/hllc.jl:17-208 and src/euler3d.jl:86-106 in the reference's order,
+//     compiled with -ffp-contract=off, so results are the reference's bits.
+//     What differs is only *where* values are computed: the cell pressure is
+//     computed once per cell (the reference recomputes it six times per
+//     interface, src/gpu_kernels.jl:69-74), and within one interface the
+//     smoothness indicator and candidate polynomials that the left and right
+//     reconstructions share (left beta1 == right beta0, left q1 == right q0,
+//     left q2 == right q1, and the first term of left beta2 == that of right
+//     beta1) are evaluated once.  Each shared value is the same expression on
+//     the same operands, so the bits do not change.
+//   * split FP32 (namespace sp): WENO5-Z weights and the HLLC flux in FP32
+//     with hardware reciprocal and square root and FMA contraction; the cell
+//     pressure (the E - rho v^2/2 cancellation) stays FP64 and is rounded to
+//     FP32 only after it is formed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#ifndef BSN_NG
+#define BSN_NG 3
+#endif
+
+namespace bsn {
+
+// ---------------------------------------------------------------------------
+// Julia Base.max / min / clamp semantics for Float64: NaN propagates, and
+// -0.0 orders below +0.0.
+
+__device__ __forceinline__ double jl_max(double x, double y) {
+    double diff = x - y;
+    double am = __builtin_signbit(diff) ? y : x;
+    return (__builtin_isnan(x) || __builtin_isnan(y)) ? diff : am;
+}
+__device__ __forceinline__ double jl_min(double x, double y) {
+    double diff = x - y;
+    double am = __builtin_signbit(diff) ? x : y;
+    return (__builtin_isnan(x) || __builtin_isnan(y)) ? diff : am;
+}
+__device__ __forceinline__ double jl_clamp(double x, double lo, double hi) {
+    return x > hi ? hi : (x < lo ? lo : x);
+}
+
+namespace ex {
+
+// src/euler3d.jl:86-90
+__device__ __forceinline__ double cell_pressure(double r, double mx, double my, double mz,
+                                                double E, double g) {
+    r = jl_max(r, 1e-30);
+    double KE = (0.5 * (mx * mx + my * my + mz * mz)) / r;
+    return jl_max((g - 1.0) * (E - KE), 0.0);
+}
+
+// src/euler3d.jl:100-106
+__device__ __forceinline__ double EL_from_P(double PL, double rL, double mxL, double myL,
+                                            double mzL, double g) {
+    double r_pos = jl_max(rL, 1e-30);
+    double m2 = mxL * mxL + myL * myL + mzL * mzL;
+    double KE_raw = (0.5 * m2) / r_pos;
+    double KE = jl_min(KE_raw, (0.5 * r_pos) * 1e4);
+    return PL / (g - 1.0) + KE;
+}
+
+// src/weno5.jl:14-21
+__device__ __forceinline__ double wenoz_combine(double b0, double b1, double b2,
+                                                double d0, double d1, double d2,
+                                                double q0, double q1, double q2) {
+    const double eps = 1e-36;
+    double tau5 = fabs(b0 - b2);
+    double t0 = tau5 / (eps + b0);
+    double t1 = tau5 / (eps + b1);
+    double t2 = tau5 / (eps + b2);
+    double a0 = d0 * (1.0 + t0 * t0);
+    double a1 = d1 * (1.0 + t1 * t1);
+    double a2 = d2 * (1.0 + t2 * t2);
+    double asum = a0 + a1 + a2;
+    double w0 = a0 / asum, w1 = a1 / asum, w2 = a2 / asum;
+    return w0 * q0 + w1 * q1 + w2 * q2;       // src/weno5.jl:40 / :60
+}
+
+// Left and right WENO5-Z values at the interface between a[2] and a[3]
+// (src/weno5.jl:25-61 applied to a[0..4] and a[1..5]).
+__device__ __forceinline__ void weno5_pair(double a0, double a1, double a2, double a3,
+                                           double a4, double a5, double& vl, double& vr) {
+    const double c13 = 13.0 / 12.0, c14 = 1.0 / 4.0;
+    // candidate polynomials
+    double qL0 = (1.0 / 3.0) * a0 + (-7.0 / 6.0) * a1 + (11.0 / 6.0) * a2;
+    double qC  = (-1.0 / 6.0) * a1 + (5.0 / 6.0) * a2 + (1.0 / 3.0) * a3;   // left q1, right q0
+    double qR  = (1.0 / 3.0) * a2 + (5.0 / 6.0) * a3 + (-1.0 / 6.0) * a4;   // left q2, right q1
+    double qR2 = (11.0 / 6.0) * a3 + (-7.0 / 6.0) * a4 + (1.0 / 3.0) * a5;
+    // smoothness indicators
+    double s, t;
+    s = a0 - 2.0 * a1 + a2;  t = a0 - 4.0 * a1 + 3.0 * a2;
+    double bL0 = c13 * (s * s) + c14 * (t * t);
+    s = a1 - 2.0 * a2 + a3;  t = a1 - a3;
+    double bC = c13 * (s * s) + c14 * (t * t);                  // left beta1, right beta0
+    s = a2 - 2.0 * a3 + a4;
+    double h = c13 * (s * s);                                    // shared first term
+    t = 3.0 * a2 - 4.0 * a3 + a4;
+    double bL2 = h + c14 * (t * t);
+    t = a2 - 4.0 * a3 + 3.0 * a4;
+    double bR1 = h + c14 * (t * t);
+    s = a3 - 2.0 * a4 + a5;  t = 3.0 * a3 - 4.0 * a4 + a5;
+    double bR2 = c13 * (s * s) + c14 * (t * t);
+    vl = wenoz_combine(bL0, bC, bL2, 1.0 / 10.0, 6.0 / 10.0, 3.0 / 10.0, qL0, qC, qR);
+    vr = wenoz_combine(bC, bR1, bR2, 3.0 / 10.0, 6.0 / 10.0, 1.0 / 10.0, qC, qR, qR2);
+}
+
+// src/hllc.jl:17-165.  Inputs in (rho, m_normal, m_t1, m_t2, E) order.
+__device__ __forceinline__ void hllc_normal(double rL, double mnL, double mt1L, double mt2L,
+                                            double EL, double rR, double mnR, double mt1R,
+                                            double mt2R, double ER, double g, double gfac,
+                                            double F[5]) {
+    if (rL <= 0.0) { rL = 1e-30; mnL = 0.0; mt1L = 0.0; mt2L = 0.0; EL = 0.0; }
+    else           { rL = jl_max(rL, 1e-30); }
+    if (rR <= 0.0) { rR = 1e-30; mnR = 0.0; mt1R = 0.0; mt2R = 0.0; ER = 0.0; }
+    else           { rR = jl_max(rR, 1e-30); }
+
+    double m2L = mnL * mnL + mt1L * mt1L + mt2L * mt2L;
+    if (m2L > (rL * rL) * 1e4) {
+        double s = (rL * 100.0) / sqrt(m2L);
+        mnL *= s; mt1L *= s; mt2L *= s;
+    }
+    double m2R = mnR * mnR + mt1R * mt1R + mt2R * mt2R;
+    if (m2R > (rR * rR) * 1e4) {
+        double s = (rR * 100.0) / sqrt(m2R);
+        mnR *= s; mt1R *= s; mt2R *= s;
+    }
+    double KE_L = jl_min((0.5 * (mnL * mnL + mt1L * mt1L + mt2L * mt2L)) / rL, (0.5 * rL) * 1e4);
+    double KE_R = jl_min((0.5 * (mnR * mnR + mt1R * mt1R + mt2R * mt2R)) / rR, (0.5 * rR) * 1e4);
+    EL = jl_max(EL, KE_L);
+    ER = jl_max(ER, KE_R);
+
+    double vnL = mnL / rL, vt1L = mt1L / rL, vt2L = mt2L / rL;
+    double vnR = mnR / rR, vt1R = mt1R / rR, vt2R = mt2R / rR;
+    double pL = (g - 1.0) * (EL - KE_L);
+    double pR = (g - 1.0) * (ER - KE_R);
+    double csL = sqrt((g * pL) / rL);
+    double csR = sqrt((g * pR) / rR);
+
+    double SL, SR;
+    if (pL > 0.0 && pR > 0.0) {
+        double r_avg = 0.5 * (rL + rR);
+        double cs_avg = 0.5 * (csL + csR);
+        double p_star = jl_max(0.0, 0.5 * (pL + pR) - ((0.5 * (vnR - vnL)) * r_avg) * cs_avg);
+        double qL = p_star > pL ? sqrt(1.0 + gfac * (p_star / pL - 1.0)) : 1.0;
+        double qR = p_star > pR ? sqrt(1.0 + gfac * (p_star / pR - 1.0)) : 1.0;
+        SL = vnL - csL * qL;
+        SR = vnR + csR * qR;
+    } else {
+        SL = jl_min(vnL - csL, vnR - csR);
+        SR = jl_max(vnL + csL, vnR + csR);
+    }
+
+    double denom = rL * (SL - vnL) - rR * (SR - vnR);
+    if (fabs(denom) < 1e-100) { F[0] = F[1] = F[2] = F[3] = F[4] = 0.0; return; }
+    double S_star = ((pR - pL) + (rL * vnL) * (SL - vnL) - (rR * vnR) * (SR - vnR)) / denom;
+
+    double FrL = mnL, FmnL = rL * (vnL * vnL) + pL, Fmt1L = mt1L * vnL, Fmt2L = mt2L * vnL;
+    double FEL = (EL + pL) * vnL;
+    if (SL >= 0.0) { F[0] = FrL; F[1] = FmnL; F[2] = Fmt1L; F[3] = Fmt2L; F[4] = FEL; return; }
+    double FrR = mnR, FmnR = rR * (vnR * vnR) + pR, Fmt1R = mt1R * vnR, Fmt2R = mt2R * vnR;
+    double FER = (ER + pR) * vnR;
+    if (SR <= 0.0) { F[0] = FrR; F[1] = FmnR; F[2] = Fmt1R; F[3] = Fmt2R; F[4] = FER; return; }
+
+    if (jl_max(rL, rR) > 3.0 * jl_min(rL, rR)) {
+        double inv_dS = 1.0 / (SR - SL);
+        double SLSR = SL * SR;
+        F[0] = (SR * FrL - SL * FrR + SLSR * (rR - rL)) * inv_dS;
+        F[1] = (SR * FmnL - SL * FmnR + SLSR * (mnR - mnL)) * inv_dS;
+        F[2] = (SR * Fmt1L - SL * Fmt1R + SLSR * (mt1R - mt1L)) * inv_dS;
+        F[3] = (SR * Fmt2L - SL * Fmt2R + SLSR * (mt2R - mt2L)) * inv_dS;
+        F[4] = (SR * FEL - SL * FER + SLSR * (ER - EL)) * inv_dS;
+        return;
+    }
+    if (S_star >= 0.0) {
+        double dL = rL * (SL - vnL);
+        double chi = dL / (SL - S_star);
+        double EsL = fabs(dL) > 1e-100 ? chi * (EL / rL + (S_star - vnL) * (S_star + pL / dL)) : 0.0;
+        F[0] = FrL + SL * (chi - rL);
+        F[1] = FmnL + SL * (chi * S_star - mnL);
+        F[2] = Fmt1L + SL * (chi * vt1L - mt1L);
+        F[3] = Fmt2L + SL * (chi * vt2L - mt2L);
+        F[4] = FEL + SL * (EsL - EL);
+    } else {
+        double dR = rR * (SR - vnR);
+        double chi = dR / (SR - S_star);
+        double EsR = fabs(dR) > 1e-100 ? chi * (ER / rR + (S_star - vnR) * (S_star + pR / dR)) : 0.0;
+        F[0] = FrR + SR * (chi - rR);
+        F[1] = FmnR + SR * (chi * S_star - mnR);
+        F[2] = Fmt1R + SR * (chi * vt1R - mt1R);
+        F[3] = Fmt2R + SR * (chi * vt2R - mt2R);
+        F[4] = FER + SR * (EsR - ER);
+    }
+}
+
+// One interface flux along direction DIR (0 = x, 1 = y, 2 = z) from the
+// six-cell stencil c[q][s] (q = rho, mx, my, mz; s = 0..5 for cells i-2..i+3
+// of the reference's src/gpu_kernels.jl:61-82) and cell pressures p[s].
+template <int DIR>
+__device__ __forceinline__ void interface_flux(const double (&c)[4][6], const double (&p)[6],
+                                               double g, double gfac, double F[5]) {
+    double L[4], R[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        weno5_pair(c[q][0], c[q][1], c[q][2], c[q][3], c[q][4], c[q][5], L[q], R[q]);
+    double P_maxL = jl_max(jl_max(jl_max(jl_max(p[0], p[1]), p[2]), p[3]), p[4]);
+    double P_maxR = jl_max(jl_max(jl_max(jl_max(p[1], p[2]), p[3]), p[4]), p[5]);
+    double PLw, PRw;
+    weno5_pair(p[0], p[1], p[2], p[3], p[4], p[5], PLw, PRw);
+    double PL = jl_clamp(PLw, 0.0, P_maxL);
+    double PR = jl_clamp(PRw, 0.0, P_maxR);
+    double EL = EL_from_P(PL, L[0], L[1], L[2], L[3], g);
+    double ER = EL_from_P(PR, R[0], R[1], R[2], R[3], g);
+    double G[5];
+    if (DIR == 0) {
+        hllc_normal(L[0], L[1], L[2], L[3], EL, R[0], R[1], R[2], R[3], ER, g, gfac, G);
+        F[0] = G[0]; F[1] = G[1]; F[2] = G[2]; F[3] = G[3]; F[4] = G[4];
+    } else if (DIR == 1) {
+        hllc_normal(L[0], L[2], L[1], L[3], EL, R[0], R[2], R[1], R[3], ER, g, gfac, G);
+        F[0] = G[0]; F[1] = G[2]; F[2] = G[1]; F[3] = G[3]; F[4] = G[4];
+    } else {
+        hllc_normal(L[0], L[3], L[1], L[2], EL, R[0], R[3], R[1], R[2], ER, g, gfac, G);
+        F[0] = G[0]; F[1] = G[2]; F[2] = G[3]; F[3] = G[1]; F[4] = G[4];
+    }
+}
+
+}  // namespace ex
+
+// ---------------------------------------------------------------------------
+// Split precision.  FP32 with hardware reciprocal (v_rcp_f32, 1 ulp) and
+// square root (v_sqrt_f32); FMA contraction is allowed in this namespace.
+namespace sp {
+
+#pragma clang fp contract(fast)
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fmaxj(float x, float y) { return x > y ? x : y; }
+__device__ __forceinline__ float fminj(float x, float y) { return x < y ? x : y; }
+
+// WENO5-Z weights.  eps is the reference's 1e-36 (a normal FP32 number),
+// raised to 1e-17*tau where needed so that (tau/(eps+beta))^2 stays below
+// 1e34 and cannot overflow FP32; whenever the raise is active the weight of
+// that stencil is already 1 - O(1e-34) in both forms.
+__device__ __forceinline__ float wenoz_combine(float b0, float b1, float b2, float d0, float d1,
+                                               float d2, float q0, float q1, float q2) {
+    float tau5 = fabsf(b0 - b2);
+    float eps = fmaxj(1e-36f, tau5 * 1e-17f);
+    float t0 = tau5 * rcp(eps + b0);
+    float t1 = tau5 * rcp(eps + b1);
+    float t2 = tau5 * rcp(eps + b2);
+    float a0 = d0 * (1.0f + t0 * t0);
+    float a1 = d1 * (1.0f + t1 * t1);
+    float a2 = d2 * (1.0f + t2 * t2);
+    float inv = rcp(a0 + a1 + a2);
+    return (a0 * q0 + a1 * q1 + a2 * q2) * inv;
+}
+
+__device__ __forceinline__ void weno5_pair(float a0, float a1, float a2, float a3, float a4,
+                                           float a5, float& vl, float& vr) {
+    const float c13 = 13.0f / 12.0f, c14 = 0.25f;
+    float qL0 = (1.0f / 3.0f) * a0 + (-7.0f / 6.0f) * a1 + (11.0f / 6.0f) * a2;
+    float qC = (-1.0f / 6.0f) * a1 + (5.0f / 6.0f) * a2 + (1.0f / 3.0f) * a3;
+    float qR = (1.0f / 3.0f) * a2 + (5.0f / 6.0f) * a3 + (-1.0f / 6.0f) * a4;
+    float qR2 = (11.0f / 6.0f) * a3 + (-7.0f / 6.0f) * a4 + (1.0f / 3.0f) * a5;
+    float s, t;
+    s = a0 - 2.0f * a1 + a2;  t = a0 - 4.0f * a1 + 3.0f * a2;
+    float bL0 = c13 * (s * s) + c14 * (t * t);
+    s = a1 - 2.0f * a2 + a3;  t = a1 - a3;
+    float bC = c13 * (s * s) + c14 * (t * t);
+    s = a2 - 2.0f * a3 + a4;
+    float h = c13 * (s * s);
+    t = 3.0f * a2 - 4.0f * a3 + a4;
+    float bL2 = h + c14 * (t * t);
+    t = a2 - 4.0f * a3 + 3.0f * a4;
+    float bR1 = h + c14 * (t * t);
+    s = a3 - 2.0f * a4 + a5;  t = 3.0f * a3 - 4.0f * a4 + a5;
+    float bR2 = c13 * (s * s) + c14 * (t * t);
+    vl = wenoz_combine(bL0, bC, bL2, 0.1f, 0.6f, 0.3f, qL0, qC, qR);
+    vr = wenoz_combine(bC, bR1, bR2, 0.3f, 0.6f, 0.1f, qC, qR, qR2);
+}
+
+// The reference's HLLC (src/hllc.jl:17-165) in FP32: same branches and
+// guards; 1e-100 thresholds become 1e-37 (the FP32 normal range).
+__device__ __forceinline__ void hllc_normal(float rL, float mnL, float mt1L, float mt2L, float EL,
+                                            float rR, float mnR, float mt1R, float mt2R, float ER,
+                                            float g, float gfac, float F[5]) {
+    if (rL <= 0.0f) { rL = 1e-30f; mnL = 0.0f; mt1L = 0.0f; mt2L = 0.0f; EL = 0.0f; }
+    else            { rL = fmaxj(rL, 1e-30f); }
+    if (rR <= 0.0f) { rR = 1e-30f; mnR = 0.0f; mt1R = 0.0f; mt2R = 0.0f; ER = 0.0f; }
+    else            { rR = fmaxj(rR, 1e-30f); }
+    float m2L = mnL * mnL + mt1L * mt1L + mt2L * mt2L;
+    if (m2L > (rL * rL) * 1e4f) {
+        float s = (rL * 100.0f) * __builtin_amdgcn_rsqf(m2L);
+        mnL *= s; mt1L *= s; mt2L *= s;
+        m2L = mnL * mnL + mt1L * mt1L + mt2L * mt2L;
+    }
+    float m2R = mnR * mnR + mt1R * mt1R + mt2R * mt2R;
+    if (m2R > (rR * rR) * 1e4f) {
+        float s = (rR * 100.0f) * __builtin_amdgcn_rsqf(m2R);
+        mnR *= s; mt1R *= s; mt2R *= s;
+        m2R = mnR * mnR + mt1R * mt1R + mt2R * mt2R;
+    }
+    float irL = rcp(rL), irR = rcp(rR);
+    float KE_L = fminj(0.5f * m2L * irL, 0.5f * rL * 1e4f);
+    float KE_R = fminj(0.5f * m2R * irR, 0.5f * rR * 1e4f);
+    EL = fmaxj(EL, KE_L);
+    ER = fmaxj(ER, KE_R);
+    float vnL = mnL * irL, vt1L = mt1L * irL, vt2L = mt2L * irL;
+    float vnR = mnR * irR, vt1R = mt1R * irR, vt2R = mt2R * irR;
+    float pL = (g - 1.0f) * (EL - KE_L);
+    float pR = (g - 1.0f) * (ER - KE_R);
+    float csL = fsqrt(g * pL * irL);
+    float csR = fsqrt(g * pR * irR);
+    float SL, SR;
+    if (pL > 0.0f && pR > 0.0f) {
+        float r_avg = 0.5f * (rL + rR);
+        float cs_avg = 0.5f * (csL + csR);
+        float p_star = fmaxj(0.0f, 0.5f * (pL + pR) - 0.5f * (vnR - vnL) * r_avg * cs_avg);
+        float qL = p_star > pL ? fsqrt(1.0f + gfac * (p_star * rcp(pL) - 1.0f)) : 1.0f;
+        float qR = p_star > pR ? fsqrt(1.0f + gfac * (p_star * rcp(pR) - 1.0f)) : 1.0f;
+        SL = vnL - csL * qL;
+        SR = vnR + csR * qR;
+    } else {
+        SL = fminj(vnL - csL, vnR - csR);
+        SR = fmaxj(vnL + csL, vnR + csR);
+    }
+    float dL = rL * (SL - vnL), dR = rR * (SR - vnR);
+    float denom = dL - dR;
+    if (fabsf(denom) < 1e-37f) { F[0] = F[1] = F[2] = F[3] = F[4] = 0.0f; return; }
+    float FmnL = rL * (vnL * vnL) + pL, FEL = (EL + pL) * vnL;
+    if (SL >= 0.0f) { F[0] = mnL; F[1] = FmnL; F[2] = mt1L * vnL; F[3] = mt2L * vnL; F[4] = FEL; return; }
+    float FmnR = rR * (vnR * vnR) + pR, FER = (ER + pR) * vnR;
+    if (SR <= 0.0f) { F[0] = mnR; F[1] = FmnR; F[2] = mt1R * vnR; F[3] = mt2R * vnR; F[4] = FER; return; }
+    if (fmaxj(rL, rR) > 3.0f * fminj(rL, rR)) {
+        float inv_dS = rcp(SR - SL);
+        float SLSR = SL * SR;
+        F[0] = (SR * mnL - SL * mnR + SLSR * (rR - rL)) * inv_dS;
+        F[1] = (SR * FmnL - SL * FmnR + SLSR * (mnR - mnL)) * inv_dS;
+        F[2] = (SR * (mt1L * vnL) - SL * (mt1R * vnR) + SLSR * (mt1R - mt1L)) * inv_dS;
+        F[3] = (SR * (mt2L * vnL) - SL * (mt2R * vnR) + SLSR * (mt2R - mt2L)) * inv_dS;
+        F[4] = (SR * FEL - SL * FER + SLSR * (ER - EL)) * inv_dS;
+        return;
+    }
+    float S_star = ((pR - pL) + mnL * (SL - vnL) - mnR * (SR - vnR)) * rcp(denom);
+    if (S_star >= 0.0f) {
+        float chi = dL * rcp(SL - S_star);
+        float EsL = fabsf(dL) > 1e-37f ? chi * (EL * irL + (S_star - vnL) * (S_star + pL * rcp(dL))) : 0.0f;
+        F[0] = mnL + SL * (chi - rL);
+        F[1] = FmnL + SL * (chi * S_star - mnL);
+        F[2] = mt1L * vnL + SL * (chi * vt1L - mt1L);
+        F[3] = mt2L * vnL + SL * (chi * vt2L - mt2L);
+        F[4] = FEL + SL * (EsL - EL);
+    } else {
+        float chi = dR * rcp(SR - S_star);
+        float EsR = fabsf(dR) > 1e-37f ? chi * (ER * irR + (S_star - vnR) * (S_star + pR * rcp(dR))) : 0.0f;
+        F[0] = mnR + SR * (chi - rR);
+        F[1] = FmnR + SR * (chi * S_star - mnR);
+        F[2] = mt1R * vnR + SR * (chi * vt1R - mt1R);
+        F[3] = mt2R * vnR + SR * (chi * vt2R - mt2R);
+        F[4] = FER + SR * (EsR - ER);
+    }
+}
+
+template <int DIR>
+__device__ __forceinline__ void interface_flux(const float (&c)[4][6], const float (&p)[6],
+                                               float g, float gfac, float ig1, float F[5]) {
+    float L[4], R[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        weno5_pair(c[q][0], c[q][1], c[q][2], c[q][3], c[q][4], c[q][5], L[q], R[q]);
+    float P_maxL = fmaxj(fmaxj(fmaxj(fmaxj(p[0], p[1]), p[2]), p[3]), p[4]);
+    float P_maxR = fmaxj(fmaxj(fmaxj(fmaxj(p[1], p[2]), p[3]), p[4]), p[5]);
+    float PLw, PRw;
+    weno5_pair(p[0], p[1], p[2], p[3], p[4], p[5], PLw, PRw);
+    float PL = fminj(fmaxj(PLw, 0.0f), P_maxL);
+    float PR = fminj(fmaxj(PRw, 0.0f), P_maxR);
+    float rpL = fmaxj(L[0], 1e-30f), rpR = fmaxj(R[0], 1e-30f);
+    float KEL = fminj(0.5f * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]) * rcp(rpL), 0.5f * rpL * 1e4f);
+    float KER = fminj(0.5f * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]) * rcp(rpR), 0.5f * rpR * 1e4f);
+    float EL = PL * ig1 + KEL;
+    float ER = PR * ig1 + KER;
+    float G[5];
+    if (DIR == 0) {
+        hllc_normal(L[0], L[1], L[2], L[3], EL, R[0], R[1], R[2], R[3], ER, g, gfac, G);
+        F[0] = G[0]; F[1] = G[1]; F[2] = G[2]; F[3] = G[3]; F[4] = G[4];
+    } else if (DIR == 1) {
+        hllc_normal(L[0], L[2], L[1], L[3], EL, R[0], R[2], R[1], R[3], ER, g, gfac, G);
+        F[0] = G[0]; F[1] = G[2]; F[2] = G[1]; F[3] = G[3]; F[4] = G[4];
+    } else {
+        hllc_normal(L[0], L[3], L[1], L[2], EL, R[0], R[3], R[1], R[2], ER, g, gfac, G);
+        F[0] = G[0]; F[1] = G[2]; F[2] = G[3]; F[3] = G[1]; F[4] = G[4];
+    }
+}
+
+}  // namespace sp
+}  // namespace bsn

@@ -104,7 +104,7 @@ def gaussian_field(n: int, seed: int, kmin: float = 2.0, kmax: float = 32.0,
     band = (kk >= kmin) & (kk <= kmax)
     amp[band] = kk[band] ** (slope / 2.0)
     spec = (rng.standard_normal(kk.shape) + 1j * rng.standard_normal(kk.shape)) * amp
-    f = np.fft.irfftn(spec, s=(n, n, n))
+    f = np.fft.irfftn(spec, s=(n, n, n), axes=(0, 1, 2))
     f -= f.mean()
     f /= f.std()
     return f

@@ -1,0 +1,167 @@
+/*
+ * bsn.h -- C ABI of libbsn_hip: the Euler right-hand side, the SSP-RK3 step,
+ * the CFL time step and the fixed-mesh-refinement step of Binary-Supernova,
+ * on AMD Instinct MI355X (gfx950).
+ *
+ * Every entry point is extern "C", takes plain pointers and scalars, returns
+ * an int status (BSN_OK = 0) and never throws; bsn_last_error() describes the
+ * last failure of the calling thread.  A call is synchronous (the device work
+ * it enqueues has finished when it returns) unless its name ends in _async.
+ *
+ * Arrays crossing the boundary use the reference's Julia layout:
+ *   state U   (5, nx+6, ny+6, nz+6) Float64, column-major, field index fastest
+ *             (src/euler3d.jl:1-8), ghost cells included;
+ *   fluxes    Fx (5, Nx+1, Ny, Nz), Fy (5, Nx, Ny+1, Nz), Fz (5, Nx, Ny, Nz+1)
+ *             (src/euler3d.jl:231-233);
+ *   phi       (nx, ny, nz) Float64 column-major, active cells only
+ *             (src/fmr_gravity.jl:146).
+ * `where` says whether such a pointer is host memory (BSN_HOST) or device
+ * memory on the grid's GPU (BSN_DEVICE).
+ *
+ * Boundary conditions are a per-axis bit mask: bit a set = periodic along
+ * axis a (x = bit 0), otherwise outflow.  The reference's bc=:outflow is 0
+ * and bc=:periodic is 7 (src/euler3d.jl:185-194).
+ */
+#ifndef BSN_H
+#define BSN_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSN_OK 0
+#define BSN_ERR_ARG 1        /* invalid argument or shape */
+#define BSN_ERR_HIP 2        /* a HIP runtime call failed */
+#define BSN_ERR_NONFINITE 3  /* non-finite value detected where checked */
+#define BSN_ERR_STATE 4      /* call not valid in the object's state */
+
+#define BSN_BC_OUTFLOW 0
+#define BSN_BC_PERIODIC 7
+
+#define BSN_PREC_FP64 0      /* all-FP64, bit-identical to the reference */
+#define BSN_PREC_SPLIT 1     /* FP32 WENO/HLLC, FP64 state, combine, reductions */
+
+#define BSN_HOST 0
+#define BSN_DEVICE 1
+
+typedef struct bsn_ctx bsn_ctx;
+typedef struct bsn_grid bsn_grid;
+typedef struct bsn_batch bsn_batch;
+typedef struct bsn_fmr bsn_fmr;
+
+/* Library and context ---------------------------------------------------- */
+int bsn_version(void);
+const char* bsn_last_error(void);
+int bsn_ctx_create(int device, bsn_ctx** out);
+int bsn_ctx_destroy(bsn_ctx* ctx);
+int bsn_ctx_synchronize(bsn_ctx* ctx);
+int bsn_ctx_info(bsn_ctx* ctx, char* buf, int len);
+
+/* A uniform grid (one problem or one FMR level) resident on the device.
+ * Replaces the (U, nx, ny, nz) argument group of the reference's functions
+ * and the FMRLevel3D struct (src/fmr3d.jl:57-71). */
+int bsn_grid_create(bsn_ctx* ctx, int64_t nx, int64_t ny, int64_t nz, int precision,
+                    bsn_grid** out);
+int bsn_grid_destroy(bsn_grid* g);
+int bsn_grid_set_state(bsn_grid* g, const double* U, int where);
+int bsn_grid_get_state(bsn_grid* g, double* U, int where);
+/* phi for the gravity source (_apply_sg_force_level!, src/fmr_gravity.jl:146);
+ * NULL removes it.  The grid keeps its own device copy. */
+int bsn_grid_set_phi(bsn_grid* g, const double* phi, int where);
+/* Geometry and physics of the grid, used by the step functions below. */
+int bsn_grid_set_params(bsn_grid* g, double dx, double dy, double dz, double gamma, int bc,
+                        double rho_floor, double P_floor);
+int bsn_grid_device_state(bsn_grid* g, double** U_soa, int64_t* field_stride, int64_t* pitch);
+
+/* Drop-ins for the reference's uniform-grid entry points ----------------- */
+/* fill_ghost_3d_outflow! / fill_ghost_3d_periodic!  (src/euler3d.jl:33,51) */
+int bsn_fill_ghost_3d(bsn_grid* g, int bc);
+/* apply_floors_3d!  (src/euler3d.jl:73) */
+int bsn_apply_floors_3d(bsn_grid* g, double rho_floor, double P_floor, double gamma);
+/* euler3d_rhs!(dU, Fx, Fy, Fz, U, nx, ny, nz, dx, dy, dz, gamma; bc, rho_floor,
+ * P_floor, fill_ghosts)  (src/euler3d.jl:182).  dU, Fx, Fy, Fz receive the
+ * reference-layout arrays (any may be NULL); U (the grid) gets its ghosts and
+ * floors as in the reference. */
+int bsn_euler3d_rhs(bsn_grid* g, double* dU, double* Fx, double* Fy, double* Fz, int where,
+                    double dx, double dy, double dz, double gamma, int bc, double rho_floor,
+                    double P_floor, int fill_ghosts);
+/* euler3d_step!(U, nx, ny, nz, dx, dy, dz, dt, gamma; bc, rho_floor, P_floor)
+ * (src/euler3d.jl:220).  If the grid has a phi, its source is added after the
+ * flux divergence of every stage, as fmr3d_step! does (src/fmr3d.jl:509-513). */
+int bsn_euler3d_step(bsn_grid* g, double dx, double dy, double dz, double dt, double gamma,
+                     int bc, double rho_floor, double P_floor);
+/* cfl_dt_3d(U, nx, ny, nz, dx, dy, dz, gamma, cfl) -> dt  (src/euler3d.jl:157) */
+int bsn_cfl_dt_3d(bsn_grid* g, double dx, double dy, double dz, double gamma, double cfl,
+                  double* dt_out);
+/* _apply_sg_force_level!(dU, U, phi, ...)  (src/fmr_gravity.jl:146): dU
+ * (reference layout) is updated in place with the grid's phi and state. */
+int bsn_apply_sg_force(bsn_grid* g, double* dU, int where, double dx, double dy, double dz);
+/* rk3_step!(U, L!, dt, tmp1, tmp2) building blocks (src/rk3.jl:14-31):
+ * bsn_rk3_save copies U to the grid's base state (Un .= U); bsn_rk3_combine
+ * applies stage 1, 2 or 3 of the Shu-Osher combination with dU (reference
+ * layout) over the whole array. */
+int bsn_rk3_save(bsn_grid* g);
+int bsn_rk3_combine(bsn_grid* g, int stage, const double* dU, int where, double dt);
+/* Conservation totals sum(U[q, active]) * dV, q = 1..5 (src/diagnostics.jl:21-58). */
+int bsn_conserved_totals(bsn_grid* g, double dV, double out[5]);
+
+/* Host-array convenience drop-ins: the reference signature on a host Julia
+ * Array U; a cached per-thread grid uploads, runs and downloads. */
+int bsn_euler3d_step_host(double* U, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                          double dz, double dt, double gamma, int bc, double rho_floor,
+                          double P_floor, int precision);
+int bsn_cfl_dt_3d_host(const double* U, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                       double dz, double gamma, double cfl, double* dt_out);
+
+/* Batches: independent grids of any shapes swept by one persistent launch
+ * per kernel over a tile work-list.  All grids share one precision and must
+ * have parameters set (bsn_grid_set_params). */
+int bsn_batch_create(bsn_ctx* ctx, int ngrids, bsn_grid* const* grids, bsn_batch** out);
+int bsn_batch_destroy(bsn_batch* b);
+/* Times every tile shape of every kernel on the current states and keeps the
+ * fastest; writes a JSON report of the choice into report (may be NULL). */
+int bsn_batch_autotune(bsn_batch* b, char* report, int report_len);
+/* Sets the flux tile shape ids explicitly (x: 0..2, y/z: 0..3). */
+int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz);
+/* One SSP-RK3 step of every grid with per-grid dt (host array of ngrids). */
+int bsn_batch_step(bsn_batch* b, const double* dt);
+/* nsteps CFL-limited steps: dt = cfl/max wave speed, computed on the device
+ * per grid each step (clipped to t_end - t when t_end > 0).  use_graph
+ * captures one whole step in a hipGraph and replays it. */
+int bsn_batch_run(bsn_batch* b, double cfl, double t_end, int nsteps, int use_graph);
+/* Per-grid simulated time and last dt after bsn_batch_run. */
+int bsn_batch_times(bsn_batch* b, double* t, double* dt);
+
+/* Fixed mesh refinement ---------------------------------------------------- */
+/* levels[0] is the root; level l+1 is nested in level l over the 1-based
+ * active box lo_hi[6l .. 6l+5] = (ci_lo, ci_hi, cj_lo, cj_hi, ck_lo, ck_hi) of
+ * level l with refinement ratio[l] (FMRGrid3D, src/fmr3d.jl:109-151).  With
+ * nlevels = 2 this is the reference's two-level grid. */
+int bsn_fmr_create(bsn_ctx* ctx, int nlevels, bsn_grid* const* levels, const int64_t* lo_hi,
+                   const int* ratio, bsn_fmr** out);
+int bsn_fmr_destroy(bsn_fmr* m);
+/* _prolong_all! / _prolong_fine_ghosts! of level l >= 1 from its parent's
+ * current state (src/fmr3d.jl:206-255). */
+int bsn_fmr_prolong_all(bsn_fmr* m, int level);
+int bsn_fmr_prolong_fine_ghosts(bsn_fmr* m, int level);
+/* _restrict! of level l >= 1 into its parent (src/fmr3d.jl:266-285). */
+int bsn_fmr_restrict(bsn_fmr* m, int level);
+/* fmr3d_step!(G, dt_c) (src/fmr3d.jl:451); each level's phi (if set) is the
+ * frozen potential of the self_gravity branch (src/fmr3d.jl:467-473). */
+int bsn_fmr3d_step(bsn_fmr* m, double dt_c);
+/* cfl_dt_fmr3d(G, cfl) (src/fmr3d.jl:616). */
+int bsn_cfl_dt_fmr3d(bsn_fmr* m, double cfl, double* dt_out);
+/* nsteps CFL-limited root steps with dt on the device; use_graph as above. */
+int bsn_fmr_run(bsn_fmr* m, double cfl, double t_end, int nsteps, int use_graph);
+int bsn_fmr_time(bsn_fmr* m, double* t, double* dt);
+
+/* Kernel timing of the last bsn_batch_run / bsn_fmr_run with profile=1:
+ * average milliseconds per launch of the flux (x, y, z) and update kernels. */
+int bsn_batch_profile(bsn_batch* b, int nsteps, double* ms_flux3, double* ms_update,
+                      double* ms_step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

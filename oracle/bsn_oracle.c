@@ -481,7 +481,8 @@ double orc_cfl_dt_3d(const double *U, int64_t nx, int64_t ny, int64_t nz,
                 if (isnan(s)) any_nan = 1;
                 else if (s > smax) smax = s;
             }
-    if (any_nan) return NAN;
+    /* maximum() propagates NaN, and NaN > 0.0 is false: dt = Inf */
+    if (any_nan) return INFINITY;
     return smax > 0.0 ? cfl / smax : INFINITY;
 }
 
