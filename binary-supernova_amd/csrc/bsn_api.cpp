@@ -764,7 +764,9 @@ extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    // the flux kernels only read U and write F: safe to repeat on the live state
+    // the flux kernels only read U and write F; the ghost fill they need is
+    // done on a saved copy so the caller's state (ghosts included) is kept
+    for (auto* g : P.grids) HIPCHK(launch_copy_state(L, g->h, g->h.U, g->h.Un));
     CHK(plan_ghost_fill(P, 0));
     std::string rep = "{";
     for (int d = 0; d < 3; ++d) {
@@ -798,6 +800,8 @@ extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
         rep += buf;
     }
     rep += "}";
+    for (auto* g : P.grids) HIPCHK(launch_copy_state(L, g->h, g->h.Un, g->h.U));
+    HIPCHK(hipStreamSynchronize(L.stream));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (report && report_len > 0) snprintf(report, (size_t)report_len, "%s", rep.c_str());
@@ -1024,7 +1028,6 @@ static int64_t ghost_total(const GridDev& d) {
 }
 
 static int fmr_prolong(bsn_fmr* m, int l, int all_cells, bool interp) {
-    const FmrDev& f = m->hf[l - 1];
     const GridDev& C = m->lev[l]->h;
     const GridDev& Pg = m->lev[l - 1]->h;
     const int64_t total = all_cells ? (int64_t)(C.nx + 2 * BSN_NG) * C.Ny * C.Nz : ghost_total(C);

@@ -237,6 +237,8 @@ __device__ __forceinline__ void interface_flux(const double (&c)[4][6], const do
 // square root (v_sqrt_f32); FMA contraction is allowed in this namespace.
 namespace sp {
 
+// A file-scope fp pragma applies to everything after it in the translation
+// unit, so contraction is switched on here and off again at the end of sp.
 #pragma clang fp contract(fast)
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -398,6 +400,8 @@ __device__ __forceinline__ void interface_flux(const float (&c)[4][6], const flo
         F[0] = G[0]; F[1] = G[2]; F[2] = G[3]; F[3] = G[1]; F[4] = G[4];
     }
 }
+
+#pragma clang fp contract(off)
 
 }  // namespace sp
 }  // namespace bsn

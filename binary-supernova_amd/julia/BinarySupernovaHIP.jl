@@ -1,0 +1,214 @@
+# BinarySupernovaHIP.jl -- Julia bindings of libbsn_hip (include/bsn.h).
+#
+# The drop-in for the reference's uniform-grid and FMR entry points on AMD
+# MI355X.  Methods keep the reference's names, positional arguments and
+# keywords (src/euler3d.jl:33-260, src/rk3.jl:14, src/fmr3d.jl:451,616,
+# src/fmr_gravity.jl:146):
+#
+#   * on a `HIPGrid` (state resident in GPU memory, the analogue of a
+#     CuArray U) every call runs on the device without host transfers;
+#   * on a plain `Array{Float64,4}` the `*_host` C entry points upload,
+#     run and download, mutating U in place like the reference.
+#
+# Not exercised in this repository's test suite: the build machine has no
+# Julia toolchain.  The C ABI underneath is tested through Python ctypes.
+module BinarySupernovaHIP
+
+export HIPGrid, euler3d_step!, euler3d_rhs!, cfl_dt_3d, apply_floors_3d!,
+       fill_ghost_3d_outflow!, fill_ghost_3d_periodic!, rk3_step!, HIPFMR,
+       fmr3d_step!, cfl_dt_fmr3d, set_phi!, download
+
+const libbsn = joinpath(@__DIR__, "..", "libbsn_hip.so")
+const NG = 3
+const BSN_HOST = Cint(0)
+
+bcmask(bc::Symbol) = bc === :periodic ? Cint(7) : bc === :outflow ? Cint(0) :
+                     throw(ArgumentError("bc must be :outflow or :periodic"))
+precision_id(p::Symbol) = p === :fp64 ? Cint(0) : p === :split ? Cint(1) :
+                          throw(ArgumentError("precision must be :fp64 or :split"))
+
+function check(rc::Cint)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:bsn_last_error, libbsn), Cstring, ()))
+    error("libbsn_hip status $rc: $msg")
+end
+
+mutable struct Context
+    handle::Ptr{Cvoid}
+end
+const _ctx = Ref{Union{Nothing,Context}}(nothing)
+function context()
+    if _ctx[] === nothing
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:bsn_ctx_create, libbsn), Cint, (Cint, Ref{Ptr{Cvoid}}), 0, h))
+        _ctx[] = Context(h[])
+    end
+    return _ctx[]
+end
+
+"""
+    HIPGrid(U; precision=:fp64)
+
+Device-resident copy of a reference-layout state `U` of size
+`(5, nx+6, ny+6, nz+6)`.
+"""
+mutable struct HIPGrid
+    handle::Ptr{Cvoid}
+    nx::Int
+    ny::Int
+    nz::Int
+    function HIPGrid(U::Array{Float64,4}; precision::Symbol = :fp64)
+        size(U, 1) == 5 || throw(ArgumentError("U must be (5, nx+6, ny+6, nz+6)"))
+        nx, ny, nz = size(U, 2) - 2NG, size(U, 3) - 2NG, size(U, 4) - 2NG
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:bsn_grid_create, libbsn), Cint,
+                    (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Ref{Ptr{Cvoid}}),
+                    context().handle, nx, ny, nz, precision_id(precision), h))
+        g = new(h[], nx, ny, nz)
+        finalizer(x -> ccall((:bsn_grid_destroy, libbsn), Cint, (Ptr{Cvoid},), x.handle), g)
+        check(ccall((:bsn_grid_set_state, libbsn), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint),
+                    g.handle, U, BSN_HOST))
+        return g
+    end
+end
+
+"""Copy the device state into a host array of the reference layout."""
+function download(g::HIPGrid, U::Array{Float64,4} = zeros(5, g.nx + 2NG, g.ny + 2NG, g.nz + 2NG))
+    check(ccall((:bsn_grid_get_state, libbsn), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint),
+                g.handle, U, BSN_HOST))
+    return U
+end
+
+"""Static potential (nx, ny, nz) added as the ρ∇Φ source after the flux
+divergence of every stage (src/gpu_kernels.jl:291-319)."""
+function set_phi!(g::HIPGrid, Φ::Union{Nothing,Array{Float64,3}})
+    check(ccall((:bsn_grid_set_phi, libbsn), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint),
+                g.handle, Φ === nothing ? C_NULL : Φ, BSN_HOST))
+end
+
+# --- src/euler3d.jl drop-ins --------------------------------------------------
+
+function fill_ghost_3d_outflow!(U::HIPGrid, nx::Int, ny::Int, nz::Int)
+    check(ccall((:bsn_fill_ghost_3d, libbsn), Cint, (Ptr{Cvoid}, Cint), U.handle, Cint(0)))
+end
+function fill_ghost_3d_periodic!(U::HIPGrid, nx::Int, ny::Int, nz::Int)
+    check(ccall((:bsn_fill_ghost_3d, libbsn), Cint, (Ptr{Cvoid}, Cint), U.handle, Cint(7)))
+end
+
+function apply_floors_3d!(U::HIPGrid, nx::Int, ny::Int, nz::Int, ρ_floor::Real, P_floor::Real,
+                          γ::Real)
+    check(ccall((:bsn_apply_floors_3d, libbsn), Cint, (Ptr{Cvoid}, Float64, Float64, Float64),
+                U.handle, ρ_floor, P_floor, γ))
+end
+
+function euler3d_rhs!(dU::Array{Float64,4}, Fx::Array{Float64,4}, Fy::Array{Float64,4},
+                      Fz::Array{Float64,4}, U::HIPGrid, nx::Int, ny::Int, nz::Int,
+                      dx::Real, dy::Real, dz::Real, γ::Real; bc::Symbol = :outflow,
+                      ρ_floor::Real = 0.0, P_floor::Real = 0.0, fill_ghosts::Bool = true)
+    check(ccall((:bsn_euler3d_rhs, libbsn), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint,
+                 Float64, Float64, Float64, Float64, Cint, Float64, Float64, Cint),
+                U.handle, dU, Fx, Fy, Fz, BSN_HOST, dx, dy, dz, γ, bcmask(bc), ρ_floor, P_floor,
+                Cint(fill_ghosts)))
+end
+
+function euler3d_step!(U::HIPGrid, nx::Int, ny::Int, nz::Int, dx::Real, dy::Real, dz::Real,
+                       dt::Real, γ::Real; bc::Symbol = :outflow, ρ_floor::Real = 0.0,
+                       P_floor::Real = 0.0)
+    check(ccall((:bsn_euler3d_step, libbsn), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Float64, Float64, Float64, Cint, Float64, Float64),
+                U.handle, dx, dy, dz, dt, γ, bcmask(bc), ρ_floor, P_floor))
+end
+
+function euler3d_step!(U::Array{Float64,4}, nx::Int, ny::Int, nz::Int, dx::Real, dy::Real,
+                       dz::Real, dt::Real, γ::Real; bc::Symbol = :outflow, ρ_floor::Real = 0.0,
+                       P_floor::Real = 0.0, precision::Symbol = :fp64)
+    check(ccall((:bsn_euler3d_step_host, libbsn), Cint,
+                (Ptr{Float64}, Int64, Int64, Int64, Float64, Float64, Float64, Float64, Float64,
+                 Cint, Float64, Float64, Cint),
+                U, nx, ny, nz, dx, dy, dz, dt, γ, bcmask(bc), ρ_floor, P_floor,
+                precision_id(precision)))
+end
+
+function cfl_dt_3d(U::HIPGrid, nx::Int, ny::Int, nz::Int, dx::Real, dy::Real, dz::Real,
+                   γ::Real, cfl::Real)
+    dt = Ref{Float64}(0.0)
+    check(ccall((:bsn_cfl_dt_3d, libbsn), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Float64, Float64, Float64, Ref{Float64}),
+                U.handle, dx, dy, dz, γ, cfl, dt))
+    return dt[]
+end
+
+function cfl_dt_3d(U::Array{Float64,4}, nx::Int, ny::Int, nz::Int, dx::Real, dy::Real,
+                   dz::Real, γ::Real, cfl::Real)
+    dt = Ref{Float64}(0.0)
+    check(ccall((:bsn_cfl_dt_3d_host, libbsn), Cint,
+                (Ptr{Float64}, Int64, Int64, Int64, Float64, Float64, Float64, Float64, Float64,
+                 Ref{Float64}),
+                U, nx, ny, nz, dx, dy, dz, γ, cfl, dt))
+    return dt[]
+end
+
+# --- src/rk3.jl ---------------------------------------------------------------
+
+"""
+    rk3_step!(U::HIPGrid, L!, dt, tmp1, tmp2)
+
+Shu-Osher SSP-RK3 on a device grid; `L!(tmp1, U)` fills the host array
+`tmp1` with L(U).  `tmp2` is unused: the base state lives on the device.
+"""
+function rk3_step!(U::HIPGrid, L!, dt::Real, tmp1::Array{Float64,4}, tmp2 = nothing)
+    check(ccall((:bsn_rk3_save, libbsn), Cint, (Ptr{Cvoid},), U.handle))
+    for stage in 1:3
+        L!(tmp1, U)
+        check(ccall((:bsn_rk3_combine, libbsn), Cint,
+                    (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Float64),
+                    U.handle, Cint(stage), tmp1, BSN_HOST, dt))
+    end
+    return nothing
+end
+
+# --- src/fmr3d.jl -------------------------------------------------------------
+
+"""
+    HIPFMR(levels, boxes, ratios)
+
+Nested levels (root first) with `boxes[l] = (ci_lo, ci_hi, cj_lo, cj_hi, ck_lo,
+ck_hi)` of level l+1 inside level l; two levels reproduce FMRGrid3D.  Each
+level's parameters must have been set with `set_params!`.
+"""
+mutable struct HIPFMR
+    handle::Ptr{Cvoid}
+    levels::Vector{HIPGrid}
+    function HIPFMR(levels::Vector{HIPGrid}, boxes::Vector{NTuple{6,Int}}, ratios::Vector{Int})
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        lh = Int64[v for b in boxes for v in b]
+        check(ccall((:bsn_fmr_create, libbsn), Cint,
+                    (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Int64}, Ptr{Cint}, Ref{Ptr{Cvoid}}),
+                    context().handle, length(levels), [g.handle for g in levels], lh,
+                    Cint.(ratios), h))
+        m = new(h[], levels)
+        finalizer(x -> ccall((:bsn_fmr_destroy, libbsn), Cint, (Ptr{Cvoid},), x.handle), m)
+        return m
+    end
+end
+
+function set_params!(g::HIPGrid, dx, dy, dz, γ; bc::Symbol = :outflow, ρ_floor = 0.0, P_floor = 0.0)
+    check(ccall((:bsn_grid_set_params, libbsn), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Float64, Float64, Cint, Float64, Float64),
+                g.handle, dx, dy, dz, γ, bcmask(bc), ρ_floor, P_floor))
+end
+
+"""fmr3d_step!(G, dt_c) -- each level's phi (set_phi!) is the frozen potential."""
+function fmr3d_step!(G::HIPFMR, dt_c::Float64)
+    check(ccall((:bsn_fmr3d_step, libbsn), Cint, (Ptr{Cvoid}, Float64), G.handle, dt_c))
+end
+
+function cfl_dt_fmr3d(G::HIPFMR, cfl::Real)
+    dt = Ref{Float64}(0.0)
+    check(ccall((:bsn_cfl_dt_fmr3d, libbsn), Cint, (Ptr{Cvoid}, Float64, Ref{Float64}),
+                G.handle, cfl, dt))
+    return dt[]
+end
+
+end # module

@@ -111,11 +111,11 @@ def gaussian_field(n: int, seed: int, kmin: float = 2.0, kmax: float = 32.0,
 
 
 def turbulent_box(n: int = 256, gamma: float = 5.0 / 3.0, mach: float = 2.0, sigma_s: float = 0.5,
-                  seed: int = 1234, p0: float = 1.0):
+                  seed: int = 1234, p0: float = 1.0, kmin: float = 2.0, kmax: float = 32.0):
     """Returns (U, (n,n,n), (dx,dx,dx)) on the unit periodic cube."""
-    s = sigma_s * gaussian_field(n, seed)
+    s = sigma_s * gaussian_field(n, seed, kmin, kmax)
     rho = np.exp(s)
-    vel = [gaussian_field(n, seed + 1 + a) for a in range(3)]
+    vel = [gaussian_field(n, seed + 1 + a, kmin, kmax) for a in range(3)]
     # scale so that sqrt(<|v|^2 rho / (gamma p)>) = mach
     v2 = vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2
     scale = mach / math.sqrt(float(np.mean(v2 * rho / (gamma * p0))))

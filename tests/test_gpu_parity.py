@@ -320,6 +320,8 @@ def test_conserved_totals():
 
 
 def test_rk3_step_generic():
+    # rk3_step! (src/rk3.jl:14-31) with L! = euler3d_rhs! against the same
+    # composition on the oracle (no floors between stages, as rk3_step!)
     nx, ny, nz = 10, 8, 6
     U = random_state(nx, ny, nz, seed=31, wild=False)
     g = bsn.Grid(nx, ny, nz).set_state(U)
@@ -330,30 +332,45 @@ def test_rk3_step_generic():
     tmp = np.empty(U.shape)
     bsn.rk3_step_(g, L_, 1e-3, tmp)
     V = U.copy()
-    O.euler3d_step(V, 0.1, 0.1, 0.1, 1e-3, 1.4, "outflow", 0.0, 0.0)
-    A = g.get_state()
-    # active cells follow euler3d_step! exactly (rk3_step! has no post-stage floors,
-    # which do not fire on this state)
-    np.testing.assert_array_equal(P.active(A), P.active(V))
+    Vn = V.copy()
+    for s in (1, 2, 3):
+        dU, _, _, _ = O.euler3d_rhs(V, 0.1, 0.1, 0.1, 1.4, "outflow", 0.0, 0.0)
+        O.rk3_combine(s, V, Vn, dU, 1e-3)
+    np.testing.assert_array_equal(g.get_state(), V)
 
 
 def test_split_precision_accuracy_turbulent_box():
     print("split turbulence", flush=True)
+    # the benchmark's spectrum (2 <= k <= 32 on 256^3 = 8+ cells per
+    # wavelength) scaled to 48^3: 2 <= k <= 6
     n, steps = 48, 20
-    U, _, (dx, _, _) = P.turbulent_box(n, seed=1234)
+    U, _, (dx, _, _) = P.turbulent_box(n, seed=1234, kmax=6.0)
     g64 = bsn.Grid(n, n, n, "fp64").set_state(U)
     g32 = bsn.Grid(n, n, n, "split").set_state(U)
     for g in (g64, g32):
         g.set_params(dx, dx, dx, 5 / 3, "periodic", 0.0, 0.0)
     m0 = g64.conserved_totals(dx ** 3)
     b64, b32 = bsn.Batch([g64]), bsn.Batch([g32])
-    b64.run(0.4, steps)
-    b32.run(0.4, steps)
+    # one step with a common dt first: the per-step flux error of the split form
+    dt = O.cfl_dt_3d(U, dx, dx, dx, 5 / 3, 0.4)
+    b64.step([dt])
+    b32.step([dt])
+    A, B = g64.get_state(), g32.get_state()
+    e1 = np.abs(P.active(A)[..., 0] - P.active(B)[..., 0]).sum() / np.abs(P.active(A)[..., 0]).sum()
+    V = U.copy()
+    O.euler3d_step(V, dx, dx, dx, dt, 5 / 3, "periodic", 0.0, 0.0)
+    print(f"one step: fp64-vs-oracle max {np.abs(A - V).max():.3e}, split L1 {e1:.3e}", flush=True)
+    b64.run(0.4, steps - 1)
+    b32.run(0.4, steps - 1)
     A, B = g64.get_state(), g32.get_state()
     ra, rb = P.active(A)[..., 0], P.active(B)[..., 0]
     L1 = np.abs(ra - rb).sum() / np.abs(ra).sum()
     m1 = g32.conserved_totals(dx ** 3)
-    print(f"split L1 {L1:.3e} mass drift {abs(m1[0]-m0[0])/m0[0]:.3e}", flush=True)
+    m64 = g64.conserved_totals(dx ** 3)
+    print(f"split L1 {L1:.3e} mass drift split {abs(m1[0]-m0[0])/m0[0]:.3e} "
+          f"fp64 {abs(m64[0]-m0[0])/m0[0]:.3e}", flush=True)
+    assert abs(m64[0] - m0[0]) / m0[0] <= 1e-12     # no density floor fired
+    assert e1 <= 1e-6
     assert L1 <= 1e-5
     assert abs(m1[0] - m0[0]) / m0[0] <= 1e-12
 
