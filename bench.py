@@ -187,27 +187,186 @@ def bench_sedov(bsn, args):
     dx = 1.0 / n
     U = bsn.problems.sedov_ic_3d(n, n, n, dx, dx, dx, g, E_blast=1.0 / (g - 1), rho_bg=1.0,
                                  P_floor=1e-5, x_offset=-0.5, y_offset=-0.5, z_offset=-0.5)
-    floors = (0.0, 0.0)
-    steps = args.steps or 200
+    return bench_uniform_pair(bsn, args, U, (n, n, n), (dx, dx, dx), g, "outflow", (1e-8, 1e-8),
+                              f"Sedov-Taylor {n}^3 on [-0.5,0.5]^3, gamma 5/3, E0=1 within 2.5 dx, "
+                              f"rho=1, p=1e-5, CFL 0.4, device dt", steps_default=200,
+                              headline="fp64")
+
+
+def bench_sod(bsn, args):
+    """The reference's own driver loop (test/test_sod3d.jl:107-113): host dt
+    from cfl_dt_3d, then euler3d_step!, on a device-resident grid."""
+    n = args.n or 128
+    U, dims, d = bsn.problems.sod_3d("x", n, 8, 1.4)
+    nx, ny, nz = dims
     res = {}
     for prec in ("fp64", "split"):
-        log(f"sedov {n}^3 {prec}")
-        res[prec] = run_uniform(bsn, U, (n, n, n), (dx, dx, dx), g, "outflow", floors, prec, steps,
-                                args.warmup)
-    v64 = res["fp64"]["zones"] / res["fp64"]["wall"]
-    v32 = res["split"]["zones"] / res["split"]["wall"]
+        g = bsn.Grid(nx, ny, nz, prec).set_state(U)
+        g.set_params(*d, 1.4, "outflow", 0.0, 0.0)
+        g.ctx.synchronize()
+        t0 = time.perf_counter()
+        t, steps = 0.0, 0
+        while t < 0.2:
+            dt = min(bsn.cfl_dt_3d(g, nx, ny, nz, *d, 1.4, 0.4), 0.2 - t)
+            bsn.euler3d_step_(g, nx, ny, nz, *d, dt, 1.4, bc="outflow")
+            t += dt
+            steps += 1
+        wall = time.perf_counter() - t0
+        res[prec] = dict(wall=wall, steps=steps, state=g.get_state())
     A, B = res["fp64"]["state"], res["split"]["state"]
     ra, rb = A[3:-3, 3:-3, 3:-3, 0], B[3:-3, 3:-3, 3:-3, 0]
-    flux, upd = roofline(res["fp64"], "fp64", (n + 1) * n * n, n ** 3)
-    out = dict(metric="zone-updates/s per RK3 step (FP64)", value=v64, unit="zone-updates/s",
-               ms_per_step=1e3 * res["fp64"]["wall"] / steps, dtype="f64",
-               config=dict(workload=f"Sedov-Taylor {n}^3, gamma 5/3, E0=1, {steps} steps, CFL 0.4",
-                           grid=[n, n, n], precision="fp64"),
-               split=dict(value=v32, l1_density_vs_fp64=float(np.abs(ra - rb).sum() / np.abs(ra).sum())),
-               roofline=flux, roofline_update=upd)
+    dV = d[0] * d[1] * d[2]
+    t0, tA, tB = totals(U, dV), totals(A, dV), totals(B, dV)
+    v = {p: nx * ny * nz * res[p]["steps"] / res[p]["wall"] for p in res}
+    args.steps = res["fp64"]["steps"]
+    out = dict(metric="zone-updates/s per RK3 step (fp64)", value=v["fp64"], unit="zone-updates/s",
+               ms_per_step=1e3 * res["fp64"]["wall"] / res["fp64"]["steps"], dtype="f64",
+               config=dict(workload=f"Sod tube {nx}x{ny}x{nz}, gamma 1.4, outflow, CFL 0.4 to "
+                                    f"t=0.2 ({res['fp64']['steps']} steps), host dt loop",
+                           grid=list(dims), precision="fp64"),
+               split_value=v["split"], split_over_fp64=v["split"] / v["fp64"],
+               l1_density_split_vs_fp64=float(np.abs(ra - rb).sum() / np.abs(ra).sum()),
+               mass_drift_fp64=float(abs(tA[0] - t0[0]) / t0[0]),
+               energy_drift_fp64=float(abs(tA[4] - t0[4]) / t0[4]),
+               mass_drift_split=float(abs(tB[0] - t0[0]) / t0[0]),
+               energy_drift_split=float(abs(tB[4] - t0[4]) / t0[4]))
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(U, (n, n, n), (dx, dx, dx), g, "outflow", floors, 1)
+        out["cpu_baseline"] = cpu_baseline(U, dims, d, 1.4, "outflow", (0.0, 0.0),
+                                           res["fp64"]["steps"])
     return out
+
+
+def bench_uniform_pair(bsn, args, U, dims, d, gamma, bc, floors, workload, steps_default,
+                       headline):
+    nx, ny, nz = dims
+    dV = d[0] * d[1] * d[2]
+    t0 = totals(U, dV)
+    res = {}
+    steps = args.steps or steps_default
+    for prec in ("fp64", "split"):
+        log(f"{workload}: {prec}")
+        res[prec] = run_uniform(bsn, U, dims, d, gamma, bc, floors, prec, steps, args.warmup)
+    A, B = res["fp64"]["state"], res["split"]["state"]
+    ra, rb = A[3:-3, 3:-3, 3:-3, 0], B[3:-3, 3:-3, 3:-3, 0]
+    tA, tB = totals(A, dV), totals(B, dV)
+    v = {p: res[p]["zones"] / res[p]["wall"] for p in res}
+    r = res[headline]
+    flux, upd = roofline(r, headline, (nx + 1) * ny * nz, nx * ny * nz)
+    out = dict(metric=f"zone-updates/s per RK3 step ({headline})", value=v[headline],
+               unit="zone-updates/s", ms_per_step=1e3 * r["wall"] / steps,
+               dtype="f64" if headline == "fp64" else "f32-flux/f64-state",
+               config=dict(workload=workload, grid=list(dims), precision=headline),
+               fp64_value=v["fp64"], split_value=v["split"], split_over_fp64=v["split"] / v["fp64"],
+               l1_density_split_vs_fp64=float(np.abs(ra - rb).sum() / np.abs(ra).sum()),
+               mass_drift_fp64=float(abs(tA[0] - t0[0]) / t0[0]),
+               energy_drift_fp64=float(abs(tA[4] - t0[4]) / t0[4]),
+               mass_drift_split=float(abs(tB[0] - t0[0]) / t0[0]),
+               energy_drift_split=float(abs(tB[4] - t0[4]) / t0[4]),
+               roofline=flux, roofline_update=upd, tiles=r["tune"])
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(U, dims, d, gamma, bc, floors, 1)
+    return out
+
+
+def bench_batch(bsn, args):
+    probs = bsn.problems.batch_problems(48, seed=4321)
+    steps = args.steps or 20
+    res = {}
+    for prec in ("fp64", "split"):
+        log(f"batch of 48: {prec}")
+        grids = []
+        for p in probs:
+            g = bsn.Grid(*p["dims"], prec).set_state(p["U"])
+            g.set_params(*p["d"], p["gamma"], p["bc"], *p["floors"])
+            grids.append(g)
+        b = bsn.Batch(grids)
+        tune = b.autotune()
+        b.run(0.4, args.warmup, graph=True)
+        for g, p in zip(grids, probs):
+            g.set_state(p["U"])
+        grids[0].ctx.synchronize()
+        t0 = time.perf_counter()
+        b.run(0.4, steps, graph=True)
+        wall = time.perf_counter() - t0
+        states = [g.get_state() for g in grids]
+        res[prec] = dict(wall=wall, zones=b.zone_updates_per_step() * steps, tune=tune, states=states)
+    l1n = l1d = 0.0
+    mass = {"fp64": [0.0, 0.0], "split": [0.0, 0.0]}
+    energy = {"fp64": [0.0, 0.0], "split": [0.0, 0.0]}
+    for i, p in enumerate(probs):
+        A, B = res["fp64"]["states"][i], res["split"]["states"][i]
+        ra, rb = A[3:-3, 3:-3, 3:-3, 0], B[3:-3, 3:-3, 3:-3, 0]
+        l1n += float(np.abs(ra - rb).sum())
+        l1d += float(np.abs(ra).sum())
+        dV = p["d"][0] * p["d"][1] * p["d"][2]
+        t0 = totals(p["U"], dV)
+        for prec, S in (("fp64", A), ("split", B)):
+            t1 = totals(S, dV)
+            mass[prec][0] += t1[0] - t0[0]
+            mass[prec][1] += t0[0]
+            energy[prec][0] += t1[4] - t0[4]
+            energy[prec][1] += t0[4]
+    v = {pr: res[pr]["zones"] / res[pr]["wall"] for pr in res}
+    cells = sum(p["dims"][0] * p["dims"][1] * p["dims"][2] for p in probs)
+    return dict(metric="zone-updates/s per RK3 step (split, batched)", value=v["split"],
+                unit="zone-updates/s", ms_per_step=1e3 * res["split"]["wall"] / steps,
+                dtype="f32-flux/f64-state",
+                config=dict(workload=f"batch of 48 uneven Sod/Sedov problems ({cells} cells), "
+                                     f"persistent tile work-list, hipGraph step, device dt per problem",
+                            precision="split"),
+                fp64_value=v["fp64"], split_over_fp64=v["split"] / v["fp64"],
+                l1_density_split_vs_fp64=l1n / l1d,
+                mass_drift_fp64=abs(mass["fp64"][0]) / mass["fp64"][1],
+                energy_drift_fp64=abs(energy["fp64"][0]) / energy["fp64"][1],
+                mass_drift_split=abs(mass["split"][0]) / mass["split"][1],
+                energy_drift_split=abs(energy["split"][0]) / energy["split"][1],
+                tiles=res["split"]["tune"], tiles_fp64=res["fp64"]["tune"])
+
+
+def bench_fmr(bsn, args):
+    log("building the 3-level star")
+    st = bsn.problems.fmr_star()
+    steps = args.steps or 100
+    res = {}
+    for prec in ("fp64", "split"):
+        log(f"fmr 3 levels: {prec}")
+        dims = [lv[1] for lv in st["levels"]]
+        dxs = [lv[2] for lv in st["levels"]]
+        H = bsn.FMRHierarchy(dims, dxs, st["boxes"], st["ratios"], st["gamma"], "outflow",
+                             *st["floors"], precision=prec)
+        for g, lv in zip(H.levels, st["levels"]):
+            g.set_state(lv[0])
+        H.prolong_fine_ghosts(1)
+        H.prolong_fine_ghosts(2)
+        H.set_phis(st["phis"])
+        init = [g.get_state() for g in H.levels]
+        H.run(0.4, args.warmup, graph=True)
+        for g, U in zip(H.levels, init):
+            g.set_state(U)
+        H.levels[0].ctx.synchronize()
+        t0 = time.perf_counter()
+        H.run(0.4, steps, graph=True)
+        wall = time.perf_counter() - t0
+        res[prec] = dict(wall=wall, zones=H.zone_updates_per_step() * steps,
+                         states=[g.get_state() for g in H.levels], init=init, dxs=dxs)
+    v = {pr: res[pr]["zones"] / res[pr]["wall"] for pr in res}
+    A, B = res["fp64"]["states"][0], res["split"]["states"][0]
+    ra, rb = A[3:-3, 3:-3, 3:-3, 0], B[3:-3, 3:-3, 3:-3, 0]
+    dV = np.prod(res["fp64"]["dxs"][0])
+    t0 = totals(res["fp64"]["init"][0], dV)
+    tA, tB = totals(A, dV), totals(B, dV)
+    return dict(metric="zone-updates/s per RK3 step (split, 3-level FMR)", value=v["split"],
+                unit="zone-updates/s", ms_per_step=1e3 * res["split"]["wall"] / steps,
+                dtype="f32-flux/f64-state",
+                config=dict(workload="3-level FMR (192^3 + 96^3 + 128x128x96, ratio 2) n=3 "
+                                     "polytrope + two Plummer point masses + self-potential phi",
+                            precision="split"),
+                fp64_value=v["fp64"], split_over_fp64=v["split"] / v["fp64"],
+                l1_density_split_vs_fp64_base=float(np.abs(ra - rb).sum() / np.abs(ra).sum()),
+                mass_drift_fp64_base=float(abs(tA[0] - t0[0]) / t0[0]),
+                mass_drift_split_base=float(abs(tB[0] - t0[0]) / t0[0]),
+                energy_drift_fp64_base=float(abs(tA[4] - t0[4]) / t0[4]),
+                energy_drift_split_base=float(abs(tB[4] - t0[4]) / t0[4]))
 
 
 def main():
@@ -215,7 +374,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="turb256", choices=["turb256", "sedov128"])
+    ap.add_argument("--config", default="turb256",
+                    choices=["turb256", "sedov128", "sod128", "batch48", "fmr3"])
     ap.add_argument("--n", type=int, default=None, help="override the grid size")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -229,9 +389,18 @@ def main():
     if args.config == "turb256":
         args.steps = args.steps or 50
         out = bench_turb(bsn, args)
-    else:
+    elif args.config == "sedov128":
         out = bench_sedov(bsn, args)
         args.steps = args.steps or 200
+    elif args.config == "batch48":
+        out = bench_batch(bsn, args)
+        args.steps = args.steps or 20
+    elif args.config == "fmr3":
+        out = bench_fmr(bsn, args)
+        args.steps = args.steps or 100
+    else:
+        args.steps = args.steps or 100
+        out = bench_sod(bsn, args)
     line = dict(metric=out.pop("metric"), value=out.pop("value"), unit=out.pop("unit"), n_gpus=1,
                 steps=args.steps, warmup=args.warmup, ms_per_step=out.pop("ms_per_step"),
                 higher_is_better=True, scaling="weak", vs_baseline=None, dtype=out.pop("dtype"),

@@ -258,3 +258,97 @@ def prolong_scalar_trilinear(phi_c: np.ndarray, lo: tuple, ratio: int, nf: tuple
             + (1 - fx) * fy * (1 - fz) * P(0, 1, 0) + fx * fy * (1 - fz) * P(1, 1, 0)
             + (1 - fx) * (1 - fy) * fz * P(0, 0, 1) + fx * (1 - fy) * fz * P(1, 0, 1)
             + (1 - fx) * fy * fz * P(0, 1, 1) + fx * fy * fz * P(1, 1, 1))
+
+
+# ---------------------------------------------------------------------------
+# Batch of independent problems of uneven sizes
+
+BATCH_SIZES = (32, 48, 64, 96, 128)
+
+
+def batch_problems(count: int = 48, seed: int = 4321):
+    """`count` independent problems on the unit cube: per-axis sizes drawn
+    from BATCH_SIZES; a Sod tube along a random axis (left/right rho and p
+    log-uniform on [0.1, 10], at rest) or a Sedov blast (E0 log-uniform on
+    [0.1, 10] deposited within 2.5 dx of the centre, ambient rho = 1,
+    p = 1e-5); gamma 1.4 or 5/3; outflow boundaries.
+    Returns a list of dicts (U, dims, d, gamma, kind, bc, floors)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nx, ny, nz = (int(v) for v in rng.choice(BATCH_SIZES, size=3))
+        gamma = float(rng.choice([1.4, 5.0 / 3.0]))
+        d = (1.0 / nx, 1.0 / ny, 1.0 / nz)
+        if rng.random() < 0.5:
+            axis = int(rng.integers(3))
+            rL, rR, pL, pR = (float(v) for v in 10.0 ** rng.uniform(-1, 1, size=4))
+            U = empty_state(nx, ny, nz)
+            A = active(U)
+            n_axis = (nx, ny, nz)[axis]
+            c = (np.arange(1, n_axis + 1) - 0.5) * d[axis] - 0.5
+            left = c < 0.0
+            rho = np.where(left, rL, rR)
+            E = np.where(left, pL, pR) / (gamma - 1)
+            shp = [1, 1, 1]
+            shp[2 - axis] = n_axis
+            A[..., 0] = rho.reshape(shp)
+            A[..., 4] = E.reshape(shp)
+            kind = f"sod-{'xyz'[axis]}"
+        else:
+            E0 = float(10.0 ** rng.uniform(-1, 1))
+            U = sedov_ic_3d(nx, ny, nz, *d, gamma, E_blast=E0 / (gamma - 1), rho_bg=1.0,
+                            P_floor=1e-5, x_offset=-0.5, y_offset=-0.5, z_offset=-0.5)
+            kind = "sedov"
+        out.append(dict(U=U, dims=(nx, ny, nz), d=d, gamma=gamma, kind=kind, bc="outflow",
+                        floors=(1e-8, 1e-8)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Three-level FMR star: n = 3 polytrope (rho_c = 1, R = 1, gamma = 4/3) in a
+# 192^3 base on [-8, 8]^3, nested 96^3 (dx/2) and 128x128x96 (dx/4) boxes,
+# atmosphere rho = 1e-10, p = 1e-12, static phi from two Plummer point masses
+# (0.5 at (+-2, 0, 0), softening 0.1) plus the star's self-potential.
+
+def fmr_star(base: int = 192, half: float = 8.0, gamma: float = 4.0 / 3.0, rho_floor: float = 1e-10,
+             P_floor: float = 1e-12, self_gravity: bool = True):
+    """Returns dict(levels=[(U, dims, d, x0)], boxes, ratios, phis, gamma, floors)."""
+    dx0 = 2 * half / base
+    lev = [((base,) * 3, (dx0,) * 3, (-half,) * 3)]
+    # level 1: [-2, 2]^3 of the base, ratio 2
+    c = base // 2
+    w1 = int(round(2.0 / dx0))                      # base cells per half-width
+    b1 = (c - w1 + 1, c + w1, c - w1 + 1, c + w1, c - w1 + 1, c + w1)
+    n1 = 2 * (2 * w1)
+    dx1 = dx0 / 2
+    lev.append(((n1,) * 3, (dx1,) * 3, (-w1 * dx0,) * 3))
+    # level 2: 128 x 128 x 96 cells at dx/4 centred in level 1
+    c1 = n1 // 2
+    hx, hz = 128 // 4, 96 // 4                       # level-1 cells per half-width
+    b2 = (c1 - hx + 1, c1 + hx, c1 - hx + 1, c1 + hx, c1 - hz + 1, c1 + hz)
+    dx2 = dx1 / 2
+    lev.append(((128, 128, 96), (dx2,) * 3, (-hx * dx1, -hx * dx1, -hz * dx1)))
+    levels = []
+    for dims, d, x0 in lev:
+        U = polytrope_ic_3d(*dims, *d, gamma, R_star=1.0, rho_c=1.0, x0=x0[0], y0=x0[1],
+                            z0=x0[2], rho_floor=rho_floor, P_floor=P_floor)
+        levels.append((U, dims, d, x0))
+    masses = [(0.5, (2.0, 0.0, 0.0), 0.1), (0.5, (-2.0, 0.0, 0.0), 0.1)]
+    phis = [point_mass_phi(*dims, *d, *x0, masses) for (_, dims, d, x0) in levels]
+    if self_gravity:
+        # composite density restricted to the base, solved once, prolonged down
+        rho = [active(U)[..., 0].copy() for (U, _, _, _) in levels]
+        for l in (2, 1):
+            bx = (b1, b2)[l - 1]
+            r = 2
+            f = rho[l]
+            nzc, nyc, nxc = f.shape[0] // r, f.shape[1] // r, f.shape[2] // r
+            avg = f.reshape(nzc, r, nyc, r, nxc, r).mean(axis=(1, 3, 5))
+            rho[l - 1][bx[4] - 1:bx[5], bx[2] - 1:bx[3], bx[0] - 1:bx[1]] = avg
+        phi_c = solve_poisson_isolated(rho[0], dx0, dx0, dx0)
+        phi_1 = prolong_scalar_trilinear(phi_c, (b1[0], b1[2], b1[4]), 2, lev[1][0])
+        phi_2 = prolong_scalar_trilinear(phi_1, (b2[0], b2[2], b2[4]), 2, lev[2][0])
+        for p, s in zip(phis, (phi_c, phi_1, phi_2)):
+            p += s
+    return dict(levels=levels, boxes=[b1, b2], ratios=[2, 2], phis=phis, gamma=gamma,
+                floors=(rho_floor, P_floor))
