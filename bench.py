@@ -30,6 +30,7 @@ import time
 import numpy as np
 
 ROOT = pathlib.Path(__file__).resolve().parent
+LEAN = False
 
 # MI355X published peaks
 HBM_PEAK_GBS = 8000.0
@@ -84,11 +85,13 @@ def run_uniform(bsn, U, dims, d, gamma, bc, floors, prec, steps, warmup, cfl=0.4
     wall = time.perf_counter() - t0
     out = g.get_state()
     # the same steps replayed as one captured hipGraph per step
-    g.set_state(U)
-    g.ctx.synchronize()
-    t0 = time.perf_counter()
-    b.run(cfl, steps, graph=True)
-    wall_graph = time.perf_counter() - t0
+    wall_graph = float("nan")
+    if not LEAN:
+        g.set_state(U)
+        g.ctx.synchronize()
+        t0 = time.perf_counter()
+        b.run(cfl, steps, graph=True)
+        wall_graph = time.perf_counter() - t0
     zones = nx * ny * nz * steps
     return dict(wall=wall, wall_graph=wall_graph, zones=zones, ms_flux=ms_flux, ms_update=ms_upd,
                 ms_step=ms_step, tune=tune, state=out, grid=g, batch=b)
@@ -143,6 +146,13 @@ def bench_turb(bsn, args):
     r64 = run_uniform(bsn, U, dims, d, gamma, bc, floors, "fp64", args.steps, args.warmup)
     log(f"turbulent box {n}^3: split run")
     r32 = run_uniform(bsn, U, dims, d, gamma, bc, floors, "split", args.steps, args.warmup)
+    # calibration for HBM counters: one plain copy of the state (U -> Un),
+    # 5 * field_stride doubles read and written (k_copy_state)
+    import ctypes as C
+    fs = C.c_int64()
+    bsn.load().bsn_grid_device_state(r32["grid"].handle, None, C.byref(fs), None)
+    bsn.load().bsn_rk3_save(r32["grid"].handle)
+    copy_bytes = 5 * fs.value * 8
     A, B = r64["state"], r32["state"]
     ra, rb = A[3:-3, 3:-3, 3:-3, 0], B[3:-3, 3:-3, 3:-3, 0]
     L1 = float(np.abs(ra - rb).sum() / np.abs(ra).sum())
@@ -174,6 +184,7 @@ def bench_turb(bsn, args):
         roofline=flux, roofline_update=upd, tiles=r32["tune"],
         kernel_ms=dict(flux_xyz=r32["ms_flux"], update=r32["ms_update"],
                        fp64_flux_xyz=r64["ms_flux"], fp64_update=r64["ms_update"]),
+        copy_calibration_bytes=copy_bytes,
     )
     if not args.no_cpu:
         log("cpu baseline")
@@ -378,7 +389,11 @@ def main():
                     choices=["turb256", "sedov128", "sod128", "batch48", "fmr3"])
     ap.add_argument("--n", type=int, default=None, help="override the grid size")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lean", action="store_true", help="skip the hipGraph replay pass "
+                    "(for profiler runs)")
     args = ap.parse_args()
+    global LEAN
+    LEAN = args.lean
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)

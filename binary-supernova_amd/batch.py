@@ -39,6 +39,11 @@ class Batch:
     def set_shapes(self, sx: int, sy: int, sz: int) -> None:
         call("bsn_batch_set_shapes", self.handle, sx, sy, sz)
 
+    def set_mode(self, inplace: bool) -> None:
+        """True: in-place step with every flux materialised; False (default):
+        out-of-place step with the z sweep fused into the stage update."""
+        call("bsn_batch_set_mode", self.handle, int(bool(inplace)))
+
     def step(self, dts) -> None:
         d = (C.c_double * len(self.grids))(*[float(v) for v in dts])
         call("bsn_batch_step", self.handle, d)

@@ -76,6 +76,7 @@ struct bsn_grid {
     double* d_stage = nullptr;     // reference-layout staging
     size_t stage_bytes = 0;
     double* d_part = nullptr;      // totals partial sums
+    double* d_work = nullptr;      // third state buffer of the out-of-place step
     Plan* plan = nullptr;          // single-grid plan
 };
 
@@ -89,6 +90,14 @@ struct Plan {
     int nft[3] = {0, 0, 0};
     Tile* d_ut = nullptr;
     int nut = 0;
+    // out-of-place step: descriptor tables of stages 1..3 and z-update tiles
+    GridDev* d_st[3] = {nullptr, nullptr, nullptr};
+    Tile* d_zt = nullptr;
+    int nzt = 0;
+    int zshape = 0;
+    int64_t* d_poff = nullptr;
+    int64_t ptotal = 0;
+    int inplace = 0;     // 1: use the in-place step with materialised z fluxes
     int64_t* d_goff = nullptr;
     int64_t gtotal = 0;
     int64_t* d_aoff = nullptr;
@@ -157,6 +166,17 @@ static std::vector<Tile> update_tiles(const std::vector<GridDev>& G) {
     return t;
 }
 
+static std::vector<Tile> zupdate_tiles(const std::vector<GridDev>& G, int kz) {
+    std::vector<Tile> t;
+    for (int g = 0; g < (int)G.size(); ++g) {
+        const GridDev& d = G[g];
+        for (int k0 = 0; k0 < d.nz; k0 += kz)
+            for (int j0 = 0; j0 < d.ny; j0 += 4)
+                for (int i0 = 0; i0 < d.nx; i0 += 64) t.push_back({g, i0, j0, k0});
+    }
+    return t;
+}
+
 template <typename T>
 static int upload_vec(T** dptr, const std::vector<T>& v) {
     if (*dptr) { HIPCHK(hipFree(*dptr)); *dptr = nullptr; }
@@ -187,11 +207,51 @@ static int plan_set_shape(Plan& P, int dir, int shape) {
     return BSN_OK;
 }
 
+static int ensure_work(bsn_grid* g) {
+    if (g->d_work) return BSN_OK;
+    HIPCHK(hipMalloc((void**)&g->d_work, 5 * g->h.fs * 8));
+    HIPCHK(hipMemset(g->d_work, 0, 5 * g->h.fs * 8));
+    return BSN_OK;
+}
+
+// Descriptor tables.  In-place table: U = Uout = the state, Un = base copy.
+// Out-of-place step with the state A, B = the Un buffer and C = the work
+// buffer:  stage 1 reads C (prepared from A), writes B;  stage 2 reads B,
+// writes C;  stage 3 reads C, writes A.  A stays the raw step-start state
+// until stage 3 overwrites it cell by cell, so it is also the base state Un.
 static int plan_refresh(Plan& P) {
     HIPCHK(hipSetDevice(P.ctx->device));
     auto G = host_descs(P);
-    if (!P.d_grids) HIPCHK(hipMalloc((void**)&P.d_grids, G.size() * sizeof(GridDev)));
-    HIPCHK(hipMemcpy(P.d_grids, G.data(), G.size() * sizeof(GridDev), hipMemcpyHostToDevice));
+    const size_t bytes = G.size() * sizeof(GridDev);
+    if (!P.d_grids) HIPCHK(hipMalloc((void**)&P.d_grids, bytes));
+    HIPCHK(hipMemcpy(P.d_grids, G.data(), bytes, hipMemcpyHostToDevice));
+    std::vector<GridDev> st[3];
+    for (size_t gi = 0; gi < G.size(); ++gi) {
+        bsn_grid* g = P.grids[gi];
+        CHK(ensure_work(g));
+        double* A = g->h.U;
+        double* B = g->h.Un;
+        double* C = g->d_work;
+        GridDev d = G[gi];
+        d.Un = A;
+        d.U = C; d.Uout = B; st[0].push_back(d);
+        d.U = B; d.Uout = C; st[1].push_back(d);
+        d.U = C; d.Uout = A; st[2].push_back(d);
+    }
+    for (int s = 0; s < 3; ++s) {
+        if (!P.d_st[s]) HIPCHK(hipMalloc((void**)&P.d_st[s], bytes));
+        HIPCHK(hipMemcpy(P.d_st[s], st[s].data(), bytes, hipMemcpyHostToDevice));
+    }
+    return BSN_OK;
+}
+
+static int plan_set_zshape(Plan& P, int shape) {
+    auto G = host_descs(P);
+    auto t = zupdate_tiles(G, zupdate_kz(shape));
+    CHK(upload_vec(&P.d_zt, t));
+    P.nzt = (int)t.size();
+    P.zshape = shape;
+    graph_drop(P.gexec, P.graph);
     return BSN_OK;
 }
 
@@ -200,19 +260,23 @@ static int plan_build(Plan& P) {
     CHK(plan_refresh(P));
     auto G = host_descs(P);
     for (int d = 0; d < 3; ++d) CHK(plan_set_shape(P, d, P.shape[d]));
+    CHK(plan_set_zshape(P, P.zshape));
     auto ut = update_tiles(G);
     CHK(upload_vec(&P.d_ut, ut));
     P.nut = (int)ut.size();
-    std::vector<int64_t> go(G.size() + 1, 0), ao(G.size() + 1, 0);
+    std::vector<int64_t> go(G.size() + 1, 0), ao(G.size() + 1, 0), po(G.size() + 1, 0);
     for (size_t g = 0; g < G.size(); ++g) {
         const GridDev& d = G[g];
         go[g + 1] = go[g] + 2 * BSN_NG * ((int64_t)d.Ny * d.Nz + (int64_t)d.nx * d.Nz + (int64_t)d.nx * d.ny);
         ao[g + 1] = ao[g] + (int64_t)d.nx * d.ny * d.nz;
+        po[g + 1] = po[g] + (int64_t)(d.nx + 2 * BSN_NG) * d.Ny * d.Nz;
     }
     CHK(upload_vec(&P.d_goff, go));
     CHK(upload_vec(&P.d_aoff, ao));
+    CHK(upload_vec(&P.d_poff, po));
     P.gtotal = go.back();
     P.atotal = ao.back();
+    P.ptotal = po.back();
     return BSN_OK;
 }
 
@@ -224,6 +288,9 @@ static void plan_free(Plan* P) {
     (void)hipFree(P->d_grids);
     for (int d = 0; d < 3; ++d) (void)hipFree(P->d_ft[d]);
     (void)hipFree(P->d_ut);
+    for (int s = 0; s < 3; ++s) (void)hipFree(P->d_st[s]);
+    (void)hipFree(P->d_zt);
+    (void)hipFree(P->d_poff);
     (void)hipFree(P->d_goff);
     (void)hipFree(P->d_aoff);
     delete P;
@@ -257,11 +324,12 @@ static void prof_mark(Plan& P, int kind) {
     P.ev_kind.push_back(kind);
 }
 
-static int plan_fluxes(Plan& P) {
+static int plan_fluxes(Plan& P, const GridDev* table = nullptr, int ndirs = 3) {
     const LaunchCtx L = P.ctx->L();
-    for (int d = 0; d < 3; ++d) {
+    if (!table) table = P.d_grids;
+    for (int d = 0; d < ndirs; ++d) {
         prof_mark(P, d);
-        HIPCHK(launch_flux(L, d, P.prec, P.shape[d], P.d_grids, P.d_ft[d], P.nft[d], 0));
+        HIPCHK(launch_flux(L, d, P.prec, P.shape[d], table, P.d_ft[d], P.nft[d], 0));
     }
     prof_mark(P, 9);
     return BSN_OK;
@@ -274,8 +342,9 @@ static int plan_update(Plan& P, int stage, int do_cfl, double* dU_out = nullptr)
     return BSN_OK;
 }
 
-static int plan_ghost_fill(Plan& P, int copy_un) {
-    HIPCHK(launch_ghost_fill(P.ctx->L(), P.d_grids, P.d_goff, (int)P.grids.size(), P.gtotal, copy_un));
+static int plan_ghost_fill(Plan& P, int copy_un, const GridDev* table = nullptr) {
+    HIPCHK(launch_ghost_fill(P.ctx->L(), table ? table : P.d_grids, P.d_goff, (int)P.grids.size(),
+                             P.gtotal, copy_un));
     return BSN_OK;
 }
 
@@ -284,13 +353,37 @@ static int plan_floors(Plan& P, int copy_un) {
     return BSN_OK;
 }
 
-static int plan_ghost_combine(Plan& P, int stage) {
-    HIPCHK(launch_ghost_combine(P.ctx->L(), P.d_grids, P.d_goff, (int)P.grids.size(), P.gtotal, stage));
+static int plan_ghost_combine(Plan& P, int stage, const GridDev* table = nullptr) {
+    HIPCHK(launch_ghost_combine(P.ctx->L(), table ? table : P.d_grids, P.d_goff,
+                                (int)P.grids.size(), P.gtotal, stage));
     return BSN_OK;
 }
 
-// One SSP-RK3 step with bc ghost fills; dt is read from each grid's scal.
+static int plan_step_inplace(Plan& P, int fused_cfl);
+
+// One SSP-RK3 step, out of place (see plan_refresh): per stage the x and y
+// sweeps write their fluxes and one fused kernel computes the z fluxes and
+// finishes every cell.  Bit-identical to the in-place step below.
 static int plan_step(Plan& P, int fused_cfl) {
+    if (P.inplace) return plan_step_inplace(P, fused_cfl);
+    const LaunchCtx L = P.ctx->L();
+    const int ng = (int)P.grids.size();
+    for (int s = 1; s <= 3; ++s) {
+        const GridDev* T = P.d_st[s - 1];
+        if (s == 1) HIPCHK(launch_prep(L, T, P.d_poff, ng, P.ptotal));
+        else CHK(plan_ghost_fill(P, 0, T));
+        CHK(plan_fluxes(P, T, 2));
+        prof_mark(P, 3);
+        HIPCHK(launch_zupdate(L, P.prec, P.zshape, T, P.d_zt, P.nzt, s, fused_cfl && s == 3));
+        prof_mark(P, 9);
+    }
+    CHK(plan_ghost_combine(P, 3, P.d_st[2]));
+    return BSN_OK;
+}
+
+// The same step in place with all three flux arrays materialised (the form
+// the FMR levels use, since their flux registers read the z faces).
+static int plan_step_inplace(Plan& P, int fused_cfl) {
     for (int s = 1; s <= 3; ++s) {
         if (s == 1) {
             CHK(plan_ghost_fill(P, 1));
@@ -407,6 +500,7 @@ extern "C" int bsn_grid_create(bsn_ctx* ctx, int64_t nx, int64_t ny, int64_t nz,
     for (int d = 0; d < 3; ++d)
         if ((e = hipMalloc(&h.F[d], 5 * h.ffs[d] * esz)) != hipSuccess) return fail(e);
     if ((e = hipMalloc((void**)&h.scal, SCAL_COUNT * 8)) != hipSuccess) return fail(e);
+    h.Uout = h.U;
     // padding and never-written flux entries stay defined
     (void)hipMemset(h.U, 0, 5 * h.fs * 8);
     (void)hipMemset(h.Un, 0, 5 * h.fs * 8);
@@ -429,6 +523,7 @@ extern "C" int bsn_grid_destroy(bsn_grid* g) {
     (void)hipFree(g->d_scratch);
     (void)hipFree(g->d_stage);
     (void)hipFree(g->d_part);
+    (void)hipFree(g->d_work);
     delete g;
     return BSN_OK;
 }
@@ -756,6 +851,13 @@ extern "C" int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz) {
     return BSN_OK;
 }
 
+extern "C" int bsn_batch_set_mode(bsn_batch* b, int inplace) {
+    if (!b || (inplace != 0 && inplace != 1)) return set_err(BSN_ERR_ARG, "bad arguments");
+    b->plan->inplace = inplace;
+    graph_drop(b->plan->gexec, b->plan->graph);
+    return BSN_OK;
+}
+
 extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
     if (!b) return set_err(BSN_ERR_ARG, "NULL batch");
     Plan& P = *b->plan;
@@ -797,6 +899,35 @@ extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
         int W, H;
         flux_shape_dims(d, best, W, H);
         snprintf(buf, sizeof buf, ", \"best\": \"%dx%d\"}", W, H);
+        rep += buf;
+    }
+    {
+        // fused z sweep + update: stage 1 of the out-of-place step writes
+        // only the work buffer B, so repeating it is harmless
+        HIPCHK(launch_prep(L, P.d_st[0], P.d_poff, (int)P.grids.size(), P.ptotal));
+        CHK(plan_fluxes(P, P.d_st[0], 2));
+        int best = 0;
+        float best_ms = 1e30f;
+        char buf[160];
+        rep += ", \"zupdate\": {";
+        for (int s = 0; s < 2; ++s) {
+            CHK(plan_set_zshape(P, s));
+            HIPCHK(launch_zupdate(L, P.prec, s, P.d_st[0], P.d_zt, P.nzt, 1, 0));
+            HIPCHK(hipEventRecord(e0, L.stream));
+            const int reps = 3;
+            for (int r = 0; r < reps; ++r)
+                HIPCHK(launch_zupdate(L, P.prec, s, P.d_st[0], P.d_zt, P.nzt, 1, 0));
+            HIPCHK(hipEventRecord(e1, L.stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            snprintf(buf, sizeof buf, "%s\"kz%d\": %.4f", s ? ", " : "", zupdate_kz(s), ms);
+            rep += buf;
+            if (ms < best_ms) { best_ms = ms; best = s; }
+        }
+        CHK(plan_set_zshape(P, best));
+        snprintf(buf, sizeof buf, ", \"best\": \"kz%d\"}", zupdate_kz(best));
         rep += buf;
     }
     rep += "}";

@@ -2,7 +2,7 @@
 // orchestration of libbsn_hip.
 //
 // Layout in HBM (one grid = one uniform patch: a problem or an FMR level):
-//   U, Un   five FP64 fields, structure of arrays.  Cell (q, i, j, k) with
+//   U, Un, Uout  five FP64 fields each, structure of arrays (same layout).  Cell (q, i, j, k) with
 //           padded indices i in [0, Nx), j in [0, Ny), k in [0, Nz) lives at
 //           q*fs + (k*Ny + j)*pitch + XOFF + i.  XOFF puts the first active
 //           cell (i = 3) on a 128-byte boundary; pitch is a multiple of 16
@@ -34,8 +34,9 @@ enum {
 };
 
 struct GridDev {
-    double* U;
-    double* Un;
+    double* U;         // state the stage reads (stencils, ghost fills)
+    double* Un;        // RK3 base state of the step
+    double* Uout;      // where the stage update writes (== U in the in-place path)
     void* F[3];
     const double* phi;
     double* scal;

@@ -127,7 +127,7 @@ __global__ void k_ghost_combine(const GridDev* __restrict__ grids, const int64_t
             if (stage == 1)      v = un + dt * 0.0;
             else if (stage == 2) v = 0.75 * un + 0.25 * u + (0.25 * dt) * 0.0;
             else                 v = (1.0 / 3.0) * un + (2.0 / 3.0) * u + ((2.0 / 3.0) * dt) * 0.0;
-            G.U[q * G.fs + d] = v;
+            G.Uout[q * G.fs + d] = v;
         }
     }
 }
@@ -172,6 +172,35 @@ __global__ void k_floors(const GridDev* __restrict__ grids, const int64_t* __res
 #pragma unroll
             for (int q = 0; q < 5; ++q) G.U[q * G.fs + c] = u[q];
         }
+    }
+}
+
+// Stage-1 preparation of the out-of-place step: every padded cell of the
+// stage input U is written from the raw step-start state Un -- ghost cells
+// take the raw value of the active cell their index maps to (ghost fill
+// before floors, euler3d.jl:189-196), active cells their floored raw value.
+// Un is not modified, so it stays the RK3 base state of the step.
+__global__ void k_prep(const GridDev* __restrict__ grids, const int64_t* __restrict__ off,
+                       int ngrids) {
+    const int64_t total = off[ngrids];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int g = find_grid(off, ngrids, e);
+        const GridDev& G = grids[g];
+        const int Nx = G.nx + 2 * NG;
+        int64_t t = e - off[g];
+        const int i = (int)(t % Nx); t /= Nx;
+        const int j = (int)(t % G.Ny);
+        const int k = (int)(t / G.Ny);
+        const int si = bc_src(i, G.nx, G.bc & 1), sj = bc_src(j, G.ny, G.bc & 2),
+                  sk = bc_src(k, G.nz, G.bc & 4);
+        const int64_t s = uidx(G, si, sj, sk), d = uidx(G, i, j, k);
+        double u[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) u[q] = G.Un[q * G.fs + s];
+        if (si == i && sj == j && sk == k) apply_floor(u, G.rho_floor, G.P_floor, G.gamma);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) G.U[q * G.fs + d] = u[q];
     }
 }
 
@@ -342,6 +371,55 @@ __device__ __forceinline__ void block_max_to(uint64_t v, unsigned long long* dst
     __syncthreads();
 }
 
+// Phi source, stage combination and floors of active cell (i, j, k) with the
+// flux divergence d (src/gpu_kernels.jl:291-319, src/euler3d.jl:244-257);
+// the result goes to Uout.  Returns the CFL speed bits of the new state.
+__device__ __forceinline__ uint64_t finish_cell(const GridDev& G, int i, int j, int k, int stage,
+                                                double (&d)[5], int do_cfl) {
+    const int64_t c = uidx(G, NG + i, NG + j, NG + k);
+    const double dt = G.scal[SCAL_DT];
+    double u[5], un[5];
+    const bool need_u = stage > 1 || G.phi != nullptr;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        un[q] = G.Un[q * G.fs + c];
+        u[q] = need_u ? G.U[q * G.fs + c] : 0.0;
+    }
+    if (G.phi) {
+        const double* ph = G.phi;
+        const int ip = min(i + 1, G.nx - 1), im = max(i - 1, 0);
+        const int jp = min(j + 1, G.ny - 1), jm = max(j - 1, 0);
+        const int kp = min(k + 1, G.nz - 1), km = max(k - 1, 0);
+        const double i2x = 0.5 * G.idx, i2y = 0.5 * G.idy, i2z = 0.5 * G.idz;
+        const double fxf = ip > im ? i2x : 2.0 * i2x;
+        const double fyf = jp > jm ? i2y : 2.0 * i2y;
+        const double fzf = kp > km ? i2z : 2.0 * i2z;
+        const int64_t row = ((int64_t)k * G.ny + j) * G.nx;
+        const double gx = (ph[row + ip] - ph[row + im]) * fxf;
+        const double gy = (ph[((int64_t)k * G.ny + jp) * G.nx + i] -
+                           ph[((int64_t)k * G.ny + jm) * G.nx + i]) * fyf;
+        const double gz = (ph[((int64_t)kp * G.ny + j) * G.nx + i] -
+                           ph[((int64_t)km * G.ny + j) * G.nx + i]) * fzf;
+        const double r = u[0];
+        const double vdotg = (u[1] * gx + u[2] * gy + u[3] * gz) / r;
+        d[1] -= r * gx;
+        d[2] -= r * gy;
+        d[3] -= r * gz;
+        d[4] -= r * vdotg;
+    }
+    double v[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        if (stage == 1)      v[q] = un[q] + dt * d[q];
+        else if (stage == 2) v[q] = 0.75 * un[q] + 0.25 * u[q] + (0.25 * dt) * d[q];
+        else                 v[q] = (1.0 / 3.0) * un[q] + (2.0 / 3.0) * u[q] + ((2.0 / 3.0) * dt) * d[q];
+    }
+    apply_floor(v, G.rho_floor, G.P_floor, G.gamma);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) G.Uout[q * G.fs + c] = v[q];
+    return do_cfl ? cfl_speed_bits(v, G) : 0;
+}
+
 template <typename T, int KZ>
 __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grids,
                                                 const Tile* __restrict__ tiles, int ntiles,
@@ -359,8 +437,6 @@ __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grid
             const T* Fy = (const T*)G.F[1];
             const T* Fz = (const T*)G.F[2];
             const int kend = min(tl.c + KZ, G.nz);
-            const double dt = G.scal[SCAL_DT];
-            const double g = G.gamma;
             double fz0[5];
             {
                 const int64_t o = ((int64_t)tl.c * G.ny + j) * G.fp[2] + i;
@@ -382,58 +458,89 @@ __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grid
                     // src/gpu_kernels.jl:165-167
                     d[q] = (-(fx1 - fx0)) * G.idx - (fy1 - fy0) * G.idy - (fz1[q] - fz0[q]) * G.idz;
                 }
-                const int64_t c = uidx(G, NG + i, NG + j, NG + k);
                 if (stage == 0) {
+                    const int64_t c = uidx(G, NG + i, NG + j, NG + k);
 #pragma unroll
                     for (int q = 0; q < 5; ++q) dU_out[q * G.fs + c] = d[q];
                 } else {
-                    double u[5], un[5];
-                    const bool need_u = stage > 1 || G.phi != nullptr;
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) {
-                        un[q] = G.Un[q * G.fs + c];
-                        u[q] = need_u ? G.U[q * G.fs + c] : 0.0;
-                    }
-                    if (G.phi) {
-                        // src/gpu_kernels.jl:297-317 (one-sided at active edges)
-                        const double* ph = G.phi;
-                        const int ip = min(i + 1, G.nx - 1), im = max(i - 1, 0);
-                        const int jp = min(j + 1, G.ny - 1), jm = max(j - 1, 0);
-                        const int kp = min(k + 1, G.nz - 1), km = max(k - 1, 0);
-                        const double i2x = 0.5 * G.idx, i2y = 0.5 * G.idy, i2z = 0.5 * G.idz;
-                        const double fxf = ip > im ? i2x : 2.0 * i2x;
-                        const double fyf = jp > jm ? i2y : 2.0 * i2y;
-                        const double fzf = kp > km ? i2z : 2.0 * i2z;
-                        const int64_t row = ((int64_t)k * G.ny + j) * G.nx;
-                        const double gx = (ph[row + ip] - ph[row + im]) * fxf;
-                        const double gy = (ph[((int64_t)k * G.ny + jp) * G.nx + i] -
-                                           ph[((int64_t)k * G.ny + jm) * G.nx + i]) * fyf;
-                        const double gz = (ph[((int64_t)kp * G.ny + j) * G.nx + i] -
-                                           ph[((int64_t)km * G.ny + j) * G.nx + i]) * fzf;
-                        const double r = u[0];
-                        const double vdotg = (u[1] * gx + u[2] * gy + u[3] * gz) / r;
-                        d[1] -= r * gx;
-                        d[2] -= r * gy;
-                        d[3] -= r * gz;
-                        d[4] -= r * vdotg;
-                    }
-                    double v[5];
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) {
-                        if (stage == 1)      v[q] = un[q] + dt * d[q];
-                        else if (stage == 2) v[q] = 0.75 * un[q] + 0.25 * u[q] + (0.25 * dt) * d[q];
-                        else                 v[q] = (1.0 / 3.0) * un[q] + (2.0 / 3.0) * u[q] + ((2.0 / 3.0) * dt) * d[q];
-                    }
-                    apply_floor(v, G.rho_floor, G.P_floor, g);
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) G.U[q * G.fs + c] = v[q];
-                    if (do_cfl) {
-                        const uint64_t b = cfl_speed_bits(v, G);
-                        smax = b > smax ? b : smax;
-                    }
+                    const uint64_t b = finish_cell(G, i, j, k, stage, d, do_cfl);
+                    smax = b > smax ? b : smax;
                 }
 #pragma unroll
                 for (int q = 0; q < 5; ++q) fz0[q] = fz1[q];
+            }
+        }
+        if (do_cfl) block_max_to(smax, (unsigned long long*)&G.scal[SCAL_SMAX]);
+    }
+}
+
+// z sweep fused with the stage update.  Tile (g, i0, j0, k0): 64 x 4 threads
+// over (i, j); each thread walks its z column from k0 for KZ cells with a
+// six-cell register window, computes the z interface fluxes in order and
+// finishes every cell as soon as both of its z faces are known, reading the
+// x and y fluxes from HBM.  The z fluxes never leave registers.  The stage
+// writes Uout != U, so columns can be walked in any order.
+template <typename T, int KZ>
+__global__ __launch_bounds__(256) void k_zupdate(const GridDev* __restrict__ grids,
+                                                 const Tile* __restrict__ tiles, int ntiles,
+                                                 int stage, int do_cfl) {
+    int t0, t1;
+    chunk(ntiles, t0, t1);
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    for (int t = t0; t < t1; ++t) {
+        const Tile tl = tiles[t];
+        const GridDev& G = grids[tl.g];
+        const int i = tl.a + lx, j = tl.b + ly, k0 = tl.c;
+        uint64_t smax = 0;
+        if (i < G.nx && j < G.ny) {
+            const int kend = min(k0 + KZ, G.nz);
+            const T* Fx = (const T*)G.F[0];
+            const T* Fy = (const T*)G.F[1];
+            T c[4][6], p[6];
+            // interface f (between active cells f-1 and f) uses padded z = f .. f+5
+            const int64_t col = uidx(G, NG + i, NG + j, 0);
+            const int64_t zs = (int64_t)G.Ny * G.pitch;
+            auto load = [&](int slot, int pz) {
+                const int64_t o = col + (int64_t)pz * zs;
+                double u[5];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + o];
+                const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) c[q][slot] = (T)u[q];
+                p[slot] = (T)pr;
+            };
+#pragma unroll
+            for (int m = 0; m < 5; ++m) load(m, k0 + m);
+            double fz0[5];
+            for (int f = k0; f <= kend; ++f) {
+                load(5, f + 5);
+                T F[5];
+                Prec<T>::template flux<2>(c, p, G, F);
+                if (f > k0) {
+                    const int k = f - 1;
+                    const int64_t ox = ((int64_t)k * G.ny + j) * G.fp[0] + i;
+                    const int64_t oy = ((int64_t)k * (G.ny + 1) + j) * G.fp[1] + i;
+                    double d[5];
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) {
+                        const double fx0 = (double)Fx[q * G.ffs[0] + ox];
+                        const double fx1 = (double)Fx[q * G.ffs[0] + ox + 1];
+                        const double fy0 = (double)Fy[q * G.ffs[1] + oy];
+                        const double fy1 = (double)Fy[q * G.ffs[1] + oy + G.fp[1]];
+                        d[q] = (-(fx1 - fx0)) * G.idx - (fy1 - fy0) * G.idy - ((double)F[q] - fz0[q]) * G.idz;
+                    }
+                    const uint64_t b = finish_cell(G, i, j, k, stage, d, do_cfl);
+                    smax = b > smax ? b : smax;
+                }
+#pragma unroll
+                for (int q = 0; q < 5; ++q) fz0[q] = (double)F[q];
+#pragma unroll
+                for (int m = 0; m < 5; ++m) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) c[q][m] = c[q][m + 1];
+                    p[m] = p[m + 1];
+                }
             }
         }
         if (do_cfl) block_max_to(smax, (unsigned long long*)&G.scal[SCAL_SMAX]);
@@ -971,6 +1078,38 @@ hipError_t launch_update(const LaunchCtx& L, int prec, const GridDev* grids, con
         hipLaunchKernelGGL((k_update<float, UPDATE_KZ>), dim3(g), dim3(256), 0, L.stream, grids,
                            tiles, ntiles, stage, do_cfl, dU_out);
     }
+    return hipGetLastError();
+}
+
+static int flat_blocks(int64_t n, int cus);
+
+template <typename T, int KZ>
+static hipError_t launch_zupdate_t(const LaunchCtx& L, const GridDev* grids, const Tile* tiles,
+                                   int ntiles, int stage, int do_cfl) {
+    const void* fn = (const void*)&k_zupdate<T, KZ>;
+    const int g = occupancy_grid(fn, 256, 0, ntiles, L.cus);
+    hipLaunchKernelGGL((k_zupdate<T, KZ>), dim3(g), dim3(256), 0, L.stream, grids, tiles, ntiles,
+                       stage, do_cfl);
+    return hipGetLastError();
+}
+
+int zupdate_kz(int shape) { return shape == 1 ? 32 : 16; }
+
+hipError_t launch_zupdate(const LaunchCtx& L, int prec, int shape, const GridDev* grids,
+                          const Tile* tiles, int ntiles, int stage, int do_cfl) {
+    if (ntiles <= 0) return hipSuccess;
+    if (prec == 0)
+        return shape == 1 ? launch_zupdate_t<double, 32>(L, grids, tiles, ntiles, stage, do_cfl)
+                          : launch_zupdate_t<double, 16>(L, grids, tiles, ntiles, stage, do_cfl);
+    return shape == 1 ? launch_zupdate_t<float, 32>(L, grids, tiles, ntiles, stage, do_cfl)
+                      : launch_zupdate_t<float, 16>(L, grids, tiles, ntiles, stage, do_cfl);
+}
+
+hipError_t launch_prep(const LaunchCtx& L, const GridDev* grids, const int64_t* off, int ngrids,
+                       int64_t total) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prep, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids, off,
+                       ngrids);
     return hipGetLastError();
 }
 

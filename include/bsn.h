@@ -124,6 +124,9 @@ int bsn_batch_destroy(bsn_batch* b);
 int bsn_batch_autotune(bsn_batch* b, char* report, int report_len);
 /* Sets the flux tile shape ids explicitly (x: 0..2, y/z: 0..3). */
 int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz);
+/* Step form: 0 (default) = out of place, z sweep fused with the stage update;
+ * 1 = in place with all fluxes materialised.  Both give the same bits. */
+int bsn_batch_set_mode(bsn_batch* b, int inplace);
 /* One SSP-RK3 step of every grid with per-grid dt (host array of ngrids). */
 int bsn_batch_step(bsn_batch* b, const double* dt);
 /* nsteps CFL-limited steps: dt = cfl/max wave speed, computed on the device

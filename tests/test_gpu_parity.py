@@ -386,3 +386,23 @@ def test_host_array_dropin():
     assert rc == 0
     O.euler3d_step(U, 0.1, 0.1, 0.1, 1e-3, 1.4, "outflow", 0.0, 0.0)
     np.testing.assert_array_equal(V, U)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "split"])
+def test_fused_and_inplace_step_forms_agree(prec):
+    print(f"step forms {prec}", flush=True)
+    shapes = [(40, 24, 36), (17, 33, 9)]
+    out = {}
+    for inplace in (False, True):
+        grids = []
+        for i, s in enumerate(shapes):
+            U = random_state(*s, seed=50 + i)
+            g = bsn.Grid(*s, prec).set_state(U)
+            g.set_params(0.05, 0.06, 0.04, 1.4, ("periodic", "outflow", "periodic"), 1e-4, 1e-4)
+            grids.append(g)
+        b = bsn.Batch(grids)
+        b.set_mode(inplace)
+        b.run(0.4, 4, graph=False)
+        out[inplace] = [g.get_state() for g in grids]
+    for a, c in zip(out[False], out[True]):
+        np.testing.assert_array_equal(a, c)
