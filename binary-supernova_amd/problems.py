@@ -322,12 +322,14 @@ def fmr_star(base: int = 192, half: float = 8.0, gamma: float = 4.0 / 3.0, rho_f
     n1 = 2 * (2 * w1)
     dx1 = dx0 / 2
     lev.append(((n1,) * 3, (dx1,) * 3, (-w1 * dx0,) * 3))
-    # level 2: 128 x 128 x 96 cells at dx/4 centred in level 1
+    # level 2: (2/3 base) x (2/3 base) x (1/2 base) cells at dx/4 centred in
+    # level 1 (128 x 128 x 96 for the 192^3 base)
     c1 = n1 // 2
-    hx, hz = 128 // 4, 96 // 4                       # level-1 cells per half-width
+    n2 = (2 * base // 3, 2 * base // 3, base // 2)
+    hx, hz = n2[0] // 4, n2[2] // 4                  # level-1 cells per half-width
     b2 = (c1 - hx + 1, c1 + hx, c1 - hx + 1, c1 + hx, c1 - hz + 1, c1 + hz)
     dx2 = dx1 / 2
-    lev.append(((128, 128, 96), (dx2,) * 3, (-hx * dx1, -hx * dx1, -hz * dx1)))
+    lev.append((n2, (dx2,) * 3, (-hx * dx1, -hx * dx1, -hz * dx1)))
     levels = []
     for dims, d, x0 in lev:
         U = polytrope_ic_3d(*dims, *d, gamma, R_star=1.0, rho_c=1.0, x0=x0[0], y0=x0[1],

@@ -306,7 +306,7 @@ def test_every_tile_shape_gives_the_same_bits():
     g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
     b = bsn.Batch([g])
     rep = b.autotune()
-    assert set(rep) == {"x", "y", "z"}
+    assert set(rep) == {"x", "y", "z", "zupdate"}
     b.step([1e-3])
     np.testing.assert_array_equal(g.get_state(), ref)
 
