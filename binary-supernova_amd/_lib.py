@@ -25,7 +25,7 @@ EXPORTS = [
     "bsn_cfl_dt_3d", "bsn_apply_sg_force", "bsn_rk3_save", "bsn_rk3_combine",
     "bsn_conserved_totals", "bsn_euler3d_step_host", "bsn_cfl_dt_3d_host", "bsn_batch_create",
     "bsn_batch_destroy", "bsn_batch_autotune", "bsn_batch_set_shapes", "bsn_batch_set_mode",
-    "bsn_batch_step",
+    "bsn_batch_set_update_shape", "bsn_batch_step",
     "bsn_batch_run", "bsn_batch_times", "bsn_fmr_create", "bsn_fmr_destroy",
     "bsn_fmr_prolong_all", "bsn_fmr_prolong_fine_ghosts", "bsn_fmr_restrict", "bsn_fmr3d_step",
     "bsn_cfl_dt_fmr3d", "bsn_fmr_run", "bsn_fmr_time", "bsn_batch_profile",
@@ -79,6 +79,7 @@ def load():
         "bsn_batch_autotune": ([_p, C.c_char_p, _i], _i),
         "bsn_batch_set_shapes": ([_p, _i, _i, _i], _i),
         "bsn_batch_set_mode": ([_p, _i], _i),
+        "bsn_batch_set_update_shape": ([_p, _i], _i),
         "bsn_batch_step": ([_p, _pd], _i),
         "bsn_batch_run": ([_p, _d, _d, _i, _i], _i),
         "bsn_batch_times": ([_p, _pd, _pd], _i),

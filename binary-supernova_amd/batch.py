@@ -39,6 +39,11 @@ class Batch:
     def set_shapes(self, sx: int, sy: int, sz: int) -> None:
         call("bsn_batch_set_shapes", self.handle, sx, sy, sz)
 
+    def set_update_shape(self, shape: int) -> None:
+        """Fused z-sweep + update kernel shape: 0/1 = 16/32 cells per thread
+        along z, 2/3 = the same with operands loaded one cell ahead."""
+        call("bsn_batch_set_update_shape", self.handle, int(shape))
+
     def set_mode(self, inplace: bool) -> None:
         """True: in-place step with every flux materialised; False (default):
         out-of-place step with the z sweep fused into the stage update."""

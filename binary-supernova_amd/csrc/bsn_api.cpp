@@ -851,6 +851,13 @@ extern "C" int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz) {
     return BSN_OK;
 }
 
+extern "C" int bsn_batch_set_update_shape(bsn_batch* b, int shape) {
+    if (!b) return set_err(BSN_ERR_ARG, "NULL batch");
+    if (shape < 0 || shape >= zupdate_shape_count()) return set_err(BSN_ERR_ARG, "shape %d", shape);
+    HIPCHK(hipSetDevice(b->plan->ctx->device));
+    return plan_set_zshape(*b->plan, shape);
+}
+
 extern "C" int bsn_batch_set_mode(bsn_batch* b, int inplace) {
     if (!b || (inplace != 0 && inplace != 1)) return set_err(BSN_ERR_ARG, "bad arguments");
     b->plan->inplace = inplace;
@@ -908,7 +915,7 @@ extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
         float best_ms = 1e30f;
         char buf[160];
         rep += ", \"zupdate\": {";
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < zupdate_shape_count(); ++s) {
             CHK(plan_set_zshape(P, s));
             HIPCHK(launch_zupdate(L, P.prec, s, P.d_st[0], P.d_zt, P.nzt, 1, 0));
             HIPCHK(hipEventRecord(e0, L.stream));
@@ -920,12 +927,13 @@ extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             ms /= reps;
-            snprintf(buf, sizeof buf, "%s\"kz%d\": %.4f", s ? ", " : "", zupdate_kz(s), ms);
+            snprintf(buf, sizeof buf, "%s\"kz%d%s\": %.4f", s ? ", " : "", zupdate_kz(s),
+                     s >= 2 ? "pf" : "", ms);
             rep += buf;
             if (ms < best_ms) { best_ms = ms; best = s; }
         }
         CHK(plan_set_zshape(P, best));
-        snprintf(buf, sizeof buf, ", \"best\": \"kz%d\"}", zupdate_kz(best));
+        snprintf(buf, sizeof buf, ", \"best\": \"kz%d%s\"}", zupdate_kz(best), best >= 2 ? "pf" : "");
         rep += buf;
     }
     rep += "}";

@@ -374,17 +374,21 @@ __device__ __forceinline__ void block_max_to(uint64_t v, unsigned long long* dst
 // Phi source, stage combination and floors of active cell (i, j, k) with the
 // flux divergence d (src/gpu_kernels.jl:291-319, src/euler3d.jl:244-257);
 // the result goes to Uout.  Returns the CFL speed bits of the new state.
-__device__ __forceinline__ uint64_t finish_cell(const GridDev& G, int i, int j, int k, int stage,
-                                                double (&d)[5], int do_cfl) {
-    const int64_t c = uidx(G, NG + i, NG + j, NG + k);
-    const double dt = G.scal[SCAL_DT];
-    double u[5], un[5];
+__device__ __forceinline__ void cell_state(const GridDev& G, int64_t c, int stage, double (&u)[5],
+                                           double (&un)[5]) {
     const bool need_u = stage > 1 || G.phi != nullptr;
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
         un[q] = G.Un[q * G.fs + c];
         u[q] = need_u ? G.U[q * G.fs + c] : 0.0;
     }
+}
+
+__device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, int j, int k, int stage,
+                                                     double (&d)[5], const double (&u)[5],
+                                                     const double (&un)[5], int do_cfl) {
+    const int64_t c = uidx(G, NG + i, NG + j, NG + k);
+    const double dt = G.scal[SCAL_DT];
     if (G.phi) {
         const double* ph = G.phi;
         const int ip = min(i + 1, G.nx - 1), im = max(i - 1, 0);
@@ -418,6 +422,13 @@ __device__ __forceinline__ uint64_t finish_cell(const GridDev& G, int i, int j, 
 #pragma unroll
     for (int q = 0; q < 5; ++q) G.Uout[q * G.fs + c] = v[q];
     return do_cfl ? cfl_speed_bits(v, G) : 0;
+}
+
+__device__ __forceinline__ uint64_t finish_cell(const GridDev& G, int i, int j, int k, int stage,
+                                                double (&d)[5], int do_cfl) {
+    double u[5], un[5];
+    cell_state(G, uidx(G, NG + i, NG + j, NG + k), stage, u, un);
+    return finish_cell_with(G, i, j, k, stage, d, u, un, do_cfl);
 }
 
 template <typename T, int KZ>
@@ -480,7 +491,7 @@ __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grid
 // finishes every cell as soon as both of its z faces are known, reading the
 // x and y fluxes from HBM.  The z fluxes never leave registers.  The stage
 // writes Uout != U, so columns can be walked in any order.
-template <typename T, int KZ>
+template <typename T, int KZ, int PF>
 __global__ __launch_bounds__(256) void k_zupdate(const GridDev* __restrict__ grids,
                                                  const Tile* __restrict__ tiles, int ntiles,
                                                  int stage, int do_cfl) {
@@ -500,37 +511,65 @@ __global__ __launch_bounds__(256) void k_zupdate(const GridDev* __restrict__ gri
             // interface f (between active cells f-1 and f) uses padded z = f .. f+5
             const int64_t col = uidx(G, NG + i, NG + j, 0);
             const int64_t zs = (int64_t)G.Ny * G.pitch;
-            auto load = [&](int slot, int pz) {
+            double nxt[5];
+            auto fetch = [&](int pz) {
                 const int64_t o = col + (int64_t)pz * zs;
-                double u[5];
 #pragma unroll
-                for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + o];
-                const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
+                for (int q = 0; q < 5; ++q) nxt[q] = G.U[q * G.fs + o];
+            };
+            auto place = [&](int slot) {
+                const double pr = ex::cell_pressure(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], G.gamma);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) c[q][slot] = (T)u[q];
+                for (int q = 0; q < 4; ++q) c[q][slot] = (T)nxt[q];
                 p[slot] = (T)pr;
             };
 #pragma unroll
-            for (int m = 0; m < 5; ++m) load(m, k0 + m);
+            for (int m = 0; m < 5; ++m) { fetch(k0 + m); place(m); }
+            fetch(k0 + 5);
             double fz0[5];
             for (int f = k0; f <= kend; ++f) {
-                load(5, f + 5);
-                T F[5];
-                Prec<T>::template flux<2>(c, p, G, F);
-                if (f > k0) {
-                    const int k = f - 1;
+                place(5);                               // cell f+5
+                if (PF && f < kend) fetch(f + 6);       // next iteration's cell, in flight during the flux
+                // operands of the update of cell k = f-1; with PF they are
+                // issued before the flux arithmetic so their latency overlaps it
+                const int k = f - 1;
+                T fx[2][5], fy[2][5];
+                double u[5], un[5];
+                if (PF && f > k0) {
                     const int64_t ox = ((int64_t)k * G.ny + j) * G.fp[0] + i;
                     const int64_t oy = ((int64_t)k * (G.ny + 1) + j) * G.fp[1] + i;
-                    double d[5];
 #pragma unroll
                     for (int q = 0; q < 5; ++q) {
-                        const double fx0 = (double)Fx[q * G.ffs[0] + ox];
-                        const double fx1 = (double)Fx[q * G.ffs[0] + ox + 1];
-                        const double fy0 = (double)Fy[q * G.ffs[1] + oy];
-                        const double fy1 = (double)Fy[q * G.ffs[1] + oy + G.fp[1]];
-                        d[q] = (-(fx1 - fx0)) * G.idx - (fy1 - fy0) * G.idy - ((double)F[q] - fz0[q]) * G.idz;
+                        fx[0][q] = Fx[q * G.ffs[0] + ox];
+                        fx[1][q] = Fx[q * G.ffs[0] + ox + 1];
+                        fy[0][q] = Fy[q * G.ffs[1] + oy];
+                        fy[1][q] = Fy[q * G.ffs[1] + oy + G.fp[1]];
                     }
-                    const uint64_t b = finish_cell(G, i, j, k, stage, d, do_cfl);
+                    cell_state(G, col + (int64_t)(NG + k) * zs, stage, u, un);
+                }
+                T F[5];
+                Prec<T>::template flux<2>(c, p, G, F);
+                if (!PF && f < kend) fetch(f + 6);
+                if (!PF && f > k0) {
+                    const int64_t ox = ((int64_t)k * G.ny + j) * G.fp[0] + i;
+                    const int64_t oy = ((int64_t)k * (G.ny + 1) + j) * G.fp[1] + i;
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) {
+                        fx[0][q] = Fx[q * G.ffs[0] + ox];
+                        fx[1][q] = Fx[q * G.ffs[0] + ox + 1];
+                        fy[0][q] = Fy[q * G.ffs[1] + oy];
+                        fy[1][q] = Fy[q * G.ffs[1] + oy + G.fp[1]];
+                    }
+                    cell_state(G, col + (int64_t)(NG + k) * zs, stage, u, un);
+                }
+                if (f > k0) {
+                    double d[5];
+#pragma unroll
+                    for (int q = 0; q < 5; ++q)
+                        d[q] = (-((double)fx[1][q] - (double)fx[0][q])) * G.idx -
+                               ((double)fy[1][q] - (double)fy[0][q]) * G.idy -
+                               ((double)F[q] - fz0[q]) * G.idz;
+                    const uint64_t b = finish_cell_with(G, i, j, k, stage, d, u, un, do_cfl);
                     smax = b > smax ? b : smax;
                 }
 #pragma unroll
@@ -1083,26 +1122,36 @@ hipError_t launch_update(const LaunchCtx& L, int prec, const GridDev* grids, con
 
 static int flat_blocks(int64_t n, int cus);
 
-template <typename T, int KZ>
+template <typename T, int KZ, int PF>
 static hipError_t launch_zupdate_t(const LaunchCtx& L, const GridDev* grids, const Tile* tiles,
                                    int ntiles, int stage, int do_cfl) {
-    const void* fn = (const void*)&k_zupdate<T, KZ>;
+    const void* fn = (const void*)&k_zupdate<T, KZ, PF>;
     const int g = occupancy_grid(fn, 256, 0, ntiles, L.cus);
-    hipLaunchKernelGGL((k_zupdate<T, KZ>), dim3(g), dim3(256), 0, L.stream, grids, tiles, ntiles,
-                       stage, do_cfl);
+    hipLaunchKernelGGL((k_zupdate<T, KZ, PF>), dim3(g), dim3(256), 0, L.stream, grids, tiles,
+                       ntiles, stage, do_cfl);
     return hipGetLastError();
 }
 
-int zupdate_kz(int shape) { return shape == 1 ? 32 : 16; }
+// shapes: 0 kz16, 1 kz32, 2 kz16 with prefetch, 3 kz32 with prefetch
+int zupdate_kz(int shape) { return (shape & 1) ? 32 : 16; }
+int zupdate_shape_count() { return 4; }
+
+template <typename T>
+static hipError_t launch_zupdate_p(const LaunchCtx& L, int shape, const GridDev* grids,
+                                   const Tile* tiles, int ntiles, int stage, int do_cfl) {
+    switch (shape) {
+        case 1: return launch_zupdate_t<T, 32, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 2: return launch_zupdate_t<T, 16, 1>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 3: return launch_zupdate_t<T, 32, 1>(L, grids, tiles, ntiles, stage, do_cfl);
+        default: return launch_zupdate_t<T, 16, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+    }
+}
 
 hipError_t launch_zupdate(const LaunchCtx& L, int prec, int shape, const GridDev* grids,
                           const Tile* tiles, int ntiles, int stage, int do_cfl) {
     if (ntiles <= 0) return hipSuccess;
-    if (prec == 0)
-        return shape == 1 ? launch_zupdate_t<double, 32>(L, grids, tiles, ntiles, stage, do_cfl)
-                          : launch_zupdate_t<double, 16>(L, grids, tiles, ntiles, stage, do_cfl);
-    return shape == 1 ? launch_zupdate_t<float, 32>(L, grids, tiles, ntiles, stage, do_cfl)
-                      : launch_zupdate_t<float, 16>(L, grids, tiles, ntiles, stage, do_cfl);
+    if (prec == 0) return launch_zupdate_p<double>(L, shape, grids, tiles, ntiles, stage, do_cfl);
+    return launch_zupdate_p<float>(L, shape, grids, tiles, ntiles, stage, do_cfl);
 }
 
 hipError_t launch_prep(const LaunchCtx& L, const GridDev* grids, const int64_t* off, int ngrids,

@@ -21,6 +21,7 @@ hipError_t launch_update(const LaunchCtx& L, int prec, const GridDev* grids, con
                          int ntiles, int stage, int do_cfl, double* dU_out);
 hipError_t launch_cfl(const LaunchCtx& L, const GridDev* grids, const Tile* tiles, int ntiles);
 int zupdate_kz(int shape);
+int zupdate_shape_count();
 hipError_t launch_zupdate(const LaunchCtx& L, int prec, int shape, const GridDev* grids,
                           const Tile* tiles, int ntiles, int stage, int do_cfl);
 hipError_t launch_prep(const LaunchCtx& L, const GridDev* grids, const int64_t* off, int ngrids,

@@ -124,6 +124,10 @@ int bsn_batch_destroy(bsn_batch* b);
 int bsn_batch_autotune(bsn_batch* b, char* report, int report_len);
 /* Sets the flux tile shape ids explicitly (x: 0..2, y/z: 0..3). */
 int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz);
+/* Sets the shape id of the fused z-sweep + stage-update kernel used by the
+ * out-of-place step: 0 = 16 cells per thread along z, 1 = 32, 2 and 3 = the
+ * same with the next cell and the update operands loaded one cell ahead. */
+int bsn_batch_set_update_shape(bsn_batch* b, int shape);
 /* Step form: 0 (default) = out of place, z sweep fused with the stage update;
  * 1 = in place with all fluxes materialised.  Both give the same bits. */
 int bsn_batch_set_mode(bsn_batch* b, int inplace);

@@ -301,6 +301,13 @@ def test_every_tile_shape_gives_the_same_bits():
             if ref is None:
                 ref = out
             np.testing.assert_array_equal(out, ref)
+    for sz in range(4):
+        g = bsn.Grid(*s).set_state(U)
+        g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
+        b = bsn.Batch([g])
+        b.set_update_shape(sz)
+        b.step([1e-3])
+        np.testing.assert_array_equal(g.get_state(), ref)
     b.autotune()
     g = bsn.Grid(*s).set_state(U)
     g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
@@ -393,7 +400,8 @@ def test_fused_and_inplace_step_forms_agree(prec):
     print(f"step forms {prec}", flush=True)
     shapes = [(40, 24, 36), (17, 33, 9)]
     out = {}
-    for inplace in (False, True):
+    # in-place form, then the fused form with every update-kernel shape
+    for form in ("inplace", 0, 1, 2, 3):
         grids = []
         for i, s in enumerate(shapes):
             U = random_state(*s, seed=50 + i)
@@ -401,8 +409,11 @@ def test_fused_and_inplace_step_forms_agree(prec):
             g.set_params(0.05, 0.06, 0.04, 1.4, ("periodic", "outflow", "periodic"), 1e-4, 1e-4)
             grids.append(g)
         b = bsn.Batch(grids)
-        b.set_mode(inplace)
+        b.set_mode(form == "inplace")
+        if form != "inplace":
+            b.set_update_shape(form)
         b.run(0.4, 4, graph=False)
-        out[inplace] = [g.get_state() for g in grids]
-    for a, c in zip(out[False], out[True]):
-        np.testing.assert_array_equal(a, c)
+        out[form] = [g.get_state() for g in grids]
+    for form in (0, 1, 2, 3):
+        for a, c in zip(out[form], out["inplace"]):
+            np.testing.assert_array_equal(a, c)
