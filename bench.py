@@ -42,10 +42,29 @@ VALU_FP32_TFLOPS = 157.3
 # root counts as one flop): WENO5-Z pair of 5 variables 620, two interface
 # energies 24, HLLC 130, cell pressure 10.
 FLOPS_PER_INTERFACE = 784
-# Algorithmic HBM bytes per cell of the fused update kernel: read 15 flux
-# words, U and Un (5 words each), write U (5 words).
+# Algorithmic HBM bytes per cell of the fused z-sweep + stage-update kernel
+# (k_zupdate), per launch: read the stage input U and the RK3 base state Un
+# (5 doubles each), the x and y fluxes (5 words each, one word per interface
+# and field: the face shared by two cells counts once) and write the new
+# state (5 doubles).  The z fluxes never leave registers.
 def update_bytes_per_cell(prec: str) -> int:
-    return 15 * (8 if prec == "fp64" else 4) + 15 * 8
+    return 3 * 5 * 8 + 2 * 5 * (8 if prec == "fp64" else 4)
+
+
+def profiled_traffic(kernel: str, prec: str):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) for the
+    default 256^3 workload, as recorded in the committed profile summary
+    (lines 'traffic <kernel> <prec> <bytes>'); None if absent."""
+    f = ROOT / "profiles" / "k_zupdate_turb256_hbm.md"
+    try:
+        for line in f.read_text().splitlines():
+            w = line.strip().strip("`").split()
+            if len(w) == 4 and w[0] == "traffic" and w[1] == kernel and w[2] == prec:
+                return float(w[3])
+    except OSError:
+        pass
+    return None
 
 
 def load_package():
@@ -119,21 +138,40 @@ def cpu_baseline(U, dims, d, gamma, bc, floors, nsteps=1):
                        f"{nx}x{ny}x{nz} state, FP64 C/OpenMP restatement, {sec:.1f} s")
 
 
-def roofline(res, prec, interfaces_per_sweep, cells):
-    """Dominant kernel of the step and its roofline position."""
-    f = res["ms_flux"]
-    flux_ms = sum(f) / 3.0
+def roofline(res, prec, interfaces_per_sweep, cells, n=None):
+    """Roofline positions.  The dominant kernel is the fused z-sweep +
+    stage-update kernel (k_zupdate, the largest share of the step): HBM-bound
+    by its algorithmic bytes, with the VALU rate of its z-flux work beside
+    it.  The x and y flux sweeps (k_flux) are VALU-bound."""
+    fl = [m for m in res["ms_flux"] if m > 0]
+    flux_ms = sum(fl) / len(fl)
     upd_ms = res["ms_update"]
     peak = VALU_FP64_TFLOPS if prec == "fp64" else VALU_FP32_TFLOPS
-    flux = dict(kernel="k_flux (x/y/z sweeps, avg)", bound="valu",
+    flux = dict(kernel="k_flux (x and y sweeps, avg)", bound="valu",
                 achieved=FLOPS_PER_INTERFACE * interfaces_per_sweep / (flux_ms * 1e-3) / 1e12,
                 peak=peak, unit="TFLOP/s", traffic=None)
     flux["frac"] = flux["achieved"] / peak
-    upd_gbs = update_bytes_per_cell(prec) * cells / (upd_ms * 1e-3) / 1e9
-    upd = dict(kernel="k_update (fused divergence + RK3 stage + floors)", bound="hbm",
-               achieved=upd_gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=upd_gbs / HBM_PEAK_GBS,
-               traffic=None)
+    nbytes = update_bytes_per_cell(prec) * cells
+    upd_gbs = nbytes / (upd_ms * 1e-3) / 1e9
+    traffic = profiled_traffic("k_zupdate", prec) if n == 256 else None
+    upd = dict(kernel="k_zupdate (z sweep + flux divergence + RK3 stage + floors + CFL)",
+               bound="hbm", achieved=upd_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
+               frac=upd_gbs / HBM_PEAK_GBS, traffic=traffic, algorithmic_bytes=nbytes,
+               ms_per_launch=upd_ms,
+               valu=dict(achieved=FLOPS_PER_INTERFACE * interfaces_per_sweep / (upd_ms * 1e-3) / 1e12,
+                         peak=peak, unit="TFLOP/s"))
+    upd["valu"]["frac"] = upd["valu"]["achieved"] / peak
+    flux["ms_per_launch"] = flux_ms
     return flux, upd
+
+
+def step_roofline(zone_updates_per_s, bytes_per_zone_update=320):
+    """Whole-step HBM position: 320 B is the algorithmic FP64-state traffic of
+    one SSP-RK3 step with perfect on-chip stencil reuse (stage 1 reads U and
+    writes U1; stages 2 and 3 read Un and U(k) and write U(k+1): 40 words)."""
+    a = zone_updates_per_s * bytes_per_zone_update / 1e9
+    return dict(bound="hbm", achieved=a, peak=HBM_PEAK_GBS, unit="GB/s", frac=a / HBM_PEAK_GBS,
+                bytes_per_zone_update=bytes_per_zone_update)
 
 
 def bench_turb(bsn, args):
@@ -161,8 +199,8 @@ def bench_turb(bsn, args):
     v32 = r32["zones"] / r32["wall"]
     cells = n ** 3
     interfaces = (n + 1) * n * n
-    flux, upd = roofline(r32, "split", interfaces, cells)
-    flux64, upd64 = roofline(r64, "fp64", interfaces, cells)
+    flux, upd = roofline(r32, "split", interfaces, cells, n)
+    flux64, upd64 = roofline(r64, "fp64", interfaces, cells, n)
     out = dict(
         metric="zone-updates/s per RK3 step (split FP32/FP64)",
         value=v32, unit="zone-updates/s",
@@ -172,8 +210,8 @@ def bench_turb(bsn, args):
                              f"P(k)~k^-11/3 (2<=k<=32), CFL 0.4, device dt",
                     grid=[n, n, n], precision="split"),
         fp64=dict(value=v64, ms_per_step=1e3 * r64["wall"] / args.steps,
-                  value_graph=r64["zones"] / r64["wall_graph"], roofline=flux64,
-                  roofline_update=upd64, tiles=r64["tune"]),
+                  value_graph=r64["zones"] / r64["wall_graph"], roofline=upd64,
+                  roofline_flux=flux64, tiles=r64["tune"]),
         split_over_fp64=v32 / v64,
         value_graph=r32["zones"] / r32["wall_graph"],
         l1_density_split_vs_fp64=L1,
@@ -181,15 +219,41 @@ def bench_turb(bsn, args):
         energy_drift_split=float(abs(tB[4] - t0[4]) / t0[4]),
         mass_drift_fp64=float(abs(tA[0] - t0[0]) / t0[0]),
         energy_drift_fp64=float(abs(tA[4] - t0[4]) / t0[4]),
-        roofline=flux, roofline_update=upd, tiles=r32["tune"],
+        roofline=upd, roofline_flux=flux, tiles=r32["tune"],
         kernel_ms=dict(flux_xyz=r32["ms_flux"], update=r32["ms_update"],
                        fp64_flux_xyz=r64["ms_flux"], fp64_update=r64["ms_update"]),
         copy_calibration_bytes=copy_bytes,
+        roofline_step=step_roofline(v32),
     )
+    out["pcie_inclusive"] = host_array_rate(bsn, U, dims, d, gamma, r32["tune"])
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(U, dims, d, gamma, bc, floors, 1)
     return out
+
+
+def host_array_rate(bsn, U, dims, d, gamma, tune, nsteps=3):
+    """The host-array drop-in (bsn_euler3d_step_host, split precision): every
+    call uploads the host state, steps it with a fixed dt and downloads it, so
+    its rate includes both PCIe transfers and the layout conversions."""
+    import ctypes as C
+    lib = bsn.load()
+    nx, ny, nz = dims
+    V = np.ascontiguousarray(U.copy())
+    dt = 1e-4
+    def step():
+        rc = lib.bsn_euler3d_step_host(C.c_void_p(V.ctypes.data), nx, ny, nz, *d, dt, gamma,
+                                       7, 0.0, 0.0, 1)
+        if rc != 0:
+            raise RuntimeError(lib.bsn_last_error().decode())
+    step()                                        # creates the cached grid
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        step()
+    sec = time.perf_counter() - t0
+    return dict(value=nx * ny * nz * nsteps / sec, unit="zone-updates/s",
+                ms_per_step=1e3 * sec / nsteps,
+                note="host-array drop-in: upload + step + download per call (PCIe both ways)")
 
 
 def bench_sedov(bsn, args):
@@ -273,7 +337,7 @@ def bench_uniform_pair(bsn, args, U, dims, d, gamma, bc, floors, workload, steps
                energy_drift_fp64=float(abs(tA[4] - t0[4]) / t0[4]),
                mass_drift_split=float(abs(tB[0] - t0[0]) / t0[0]),
                energy_drift_split=float(abs(tB[4] - t0[4]) / t0[4]),
-               roofline=flux, roofline_update=upd, tiles=r["tune"])
+               roofline=upd, roofline_flux=flux, tiles=r["tune"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(U, dims, d, gamma, bc, floors, 1)
     return out

@@ -417,3 +417,32 @@ def test_fused_and_inplace_step_forms_agree(prec):
     for form in (0, 1, 2, 3):
         for a, c in zip(out[form], out["inplace"]):
             np.testing.assert_array_equal(a, c)
+
+
+def test_golden_fixtures_bitwise():
+    # the committed vectors of tests/golden (oracle outputs on seeded inputs)
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    z = np.load(os.path.join(gold, "step_7x5x4.npz"), allow_pickle=False)
+    d, dt = tuple(float(v) for v in z["d"]), float(z["dt"])
+    nx, ny, nz = 7, 5, 4
+    for name, bc in (("outflow", "outflow"), ("periodic", "periodic"),
+                     ("mixed", ("periodic", "outflow", "periodic"))):
+        print(f"golden step {name}", flush=True)
+        g = bsn.Grid(nx, ny, nz).set_state(z[f"{name}_in"])
+        bsn.euler3d_step_(g, nx, ny, nz, *d, dt, 1.4, bc=bc, rho_floor=1e-4, P_floor=1e-4)
+        np.testing.assert_array_equal(g.get_state(), z[f"{name}_out"])
+    g = bsn.Grid(nx, ny, nz).set_state(z["cfl_in"])
+    assert bsn.cfl_dt_3d(g, nx, ny, nz, *d, 1.4, 0.4) == float(z["cfl_dt"])
+    print("golden fmr", flush=True)
+    f = np.load(os.path.join(gold, "fmr_6cube_r2.npz"), allow_pickle=False)
+    lo_hi, r = tuple(int(v) for v in f["lo_hi"]), int(f["ratio"])
+    dxc, gm = float(f["dxc"]), float(f["gamma"])
+    G = bsn.FMRGrid3D(f["coarse_in"], 6, 6, 6, dxc, dxc, dxc, *lo_hi, r, gm, bc="outflow",
+                      rho_floor=1e-10, P_floor=1e-5)
+    np.testing.assert_array_equal(G.fine.get_state(), f["fine_in"])
+    dt = bsn.cfl_dt_fmr3d(G, 0.4)
+    assert dt == float(f["dt"])
+    bsn.fmr3d_step_(G, dt)
+    np.testing.assert_array_equal(G.coarse.get_state(), f["coarse_out"])
+    np.testing.assert_array_equal(G.fine.get_state(), f["fine_out"])
