@@ -55,12 +55,12 @@ def profiled_traffic(kernel: str, prec: str):
     """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) for the
     default 256^3 workload, as recorded in the committed profile summary
-    (lines 'traffic <kernel> <prec> <bytes>'); None if absent."""
+    (lines 'traffic <kernel> <prec> <bytes> [variant]'); None if absent."""
     f = ROOT / "profiles" / "k_zupdate_turb256_hbm.md"
     try:
         for line in f.read_text().splitlines():
             w = line.strip().strip("`").split()
-            if len(w) == 4 and w[0] == "traffic" and w[1] == kernel and w[2] == prec:
+            if len(w) >= 4 and w[0] == "traffic" and w[1] == kernel and w[2] == prec:
                 return float(w[3])
     except OSError:
         pass

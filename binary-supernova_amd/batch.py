@@ -40,8 +40,10 @@ class Batch:
         call("bsn_batch_set_shapes", self.handle, sx, sy, sz)
 
     def set_update_shape(self, shape: int) -> None:
-        """Fused z-sweep + update kernel shape: 0/1 = 16/32 cells per thread
-        along z, 2/3 = the same with operands loaded one cell ahead."""
+        """Fused z-sweep + update kernel shape 0..7: even/odd = 16/32 cells per
+        thread along z; 2/3 load operands one cell ahead; 4/5 cap registers
+        for 4 waves per SIMD (split precision); 6/7 keep the window's raw
+        cells in an LDS ring."""
         call("bsn_batch_set_update_shape", self.handle, int(shape))
 
     def set_mode(self, inplace: bool) -> None:

@@ -77,6 +77,8 @@ struct bsn_grid {
     size_t stage_bytes = 0;
     double* d_part = nullptr;      // totals partial sums
     double* d_work = nullptr;      // third state buffer of the out-of-place step
+    SrcDev hs{};                   // point masses and sinks (host copy)
+    SrcDev* d_src = nullptr;
     Plan* plan = nullptr;          // single-grid plan
 };
 
@@ -95,8 +97,6 @@ struct Plan {
     Tile* d_zt = nullptr;
     int nzt = 0;
     int zshape = 0;
-    int64_t* d_poff = nullptr;
-    int64_t ptotal = 0;
     int inplace = 0;     // 1: use the in-place step with materialised z fluxes
     int64_t* d_goff = nullptr;
     int64_t gtotal = 0;
@@ -216,9 +216,12 @@ static int ensure_work(bsn_grid* g) {
 
 // Descriptor tables.  In-place table: U = Uout = the state, Un = base copy.
 // Out-of-place step with the state A, B = the Un buffer and C = the work
-// buffer:  stage 1 reads C (prepared from A), writes B;  stage 2 reads B,
-// writes C;  stage 3 reads C, writes A.  A stays the raw step-start state
-// until stage 3 overwrites it cell by cell, so it is also the base state Un.
+// buffer:  stage 1 reads A, writes B;  stage 2 reads B, writes C;  stage 3
+// reads C, writes A.  A stays the raw step-start state until stage 3
+// overwrites it cell by cell, so it is also the base state Un.  Every stage
+// reads its input with ghost cells mapped to active cells on load, and stage
+// 1 floors active cells on load, so no ghost fill or floor pass runs and A is
+// never modified before stage 3.
 static int plan_refresh(Plan& P) {
     HIPCHK(hipSetDevice(P.ctx->device));
     auto G = host_descs(P);
@@ -234,9 +237,9 @@ static int plan_refresh(Plan& P) {
         double* C = g->d_work;
         GridDev d = G[gi];
         d.Un = A;
-        d.U = C; d.Uout = B; st[0].push_back(d);
-        d.U = B; d.Uout = C; st[1].push_back(d);
-        d.U = C; d.Uout = A; st[2].push_back(d);
+        d.lmode = 3; d.U = A; d.Uout = B; st[0].push_back(d);
+        d.lmode = 1; d.U = B; d.Uout = C; st[1].push_back(d);
+        d.lmode = 1; d.U = C; d.Uout = A; st[2].push_back(d);
     }
     for (int s = 0; s < 3; ++s) {
         if (!P.d_st[s]) HIPCHK(hipMalloc((void**)&P.d_st[s], bytes));
@@ -264,19 +267,16 @@ static int plan_build(Plan& P) {
     auto ut = update_tiles(G);
     CHK(upload_vec(&P.d_ut, ut));
     P.nut = (int)ut.size();
-    std::vector<int64_t> go(G.size() + 1, 0), ao(G.size() + 1, 0), po(G.size() + 1, 0);
+    std::vector<int64_t> go(G.size() + 1, 0), ao(G.size() + 1, 0);
     for (size_t g = 0; g < G.size(); ++g) {
         const GridDev& d = G[g];
         go[g + 1] = go[g] + 2 * BSN_NG * ((int64_t)d.Ny * d.Nz + (int64_t)d.nx * d.Nz + (int64_t)d.nx * d.ny);
         ao[g + 1] = ao[g] + (int64_t)d.nx * d.ny * d.nz;
-        po[g + 1] = po[g] + (int64_t)(d.nx + 2 * BSN_NG) * d.Ny * d.Nz;
     }
     CHK(upload_vec(&P.d_goff, go));
     CHK(upload_vec(&P.d_aoff, ao));
-    CHK(upload_vec(&P.d_poff, po));
     P.gtotal = go.back();
     P.atotal = ao.back();
-    P.ptotal = po.back();
     return BSN_OK;
 }
 
@@ -290,7 +290,6 @@ static void plan_free(Plan* P) {
     (void)hipFree(P->d_ut);
     for (int s = 0; s < 3; ++s) (void)hipFree(P->d_st[s]);
     (void)hipFree(P->d_zt);
-    (void)hipFree(P->d_poff);
     (void)hipFree(P->d_goff);
     (void)hipFree(P->d_aoff);
     delete P;
@@ -367,11 +366,8 @@ static int plan_step_inplace(Plan& P, int fused_cfl);
 static int plan_step(Plan& P, int fused_cfl) {
     if (P.inplace) return plan_step_inplace(P, fused_cfl);
     const LaunchCtx L = P.ctx->L();
-    const int ng = (int)P.grids.size();
     for (int s = 1; s <= 3; ++s) {
         const GridDev* T = P.d_st[s - 1];
-        if (s == 1) HIPCHK(launch_prep(L, T, P.d_poff, ng, P.ptotal));
-        else CHK(plan_ghost_fill(P, 0, T));
         CHK(plan_fluxes(P, T, 2));
         prof_mark(P, 3);
         HIPCHK(launch_zupdate(L, P.prec, P.zshape, T, P.d_zt, P.nzt, s, fused_cfl && s == 3));
@@ -524,6 +520,7 @@ extern "C" int bsn_grid_destroy(bsn_grid* g) {
     (void)hipFree(g->d_stage);
     (void)hipFree(g->d_part);
     (void)hipFree(g->d_work);
+    (void)hipFree(g->d_src);
     delete g;
     return BSN_OK;
 }
@@ -606,6 +603,124 @@ static int grid_changed(bsn_grid* g) {
     return BSN_OK;
 }
 
+// Uploads the grid's point masses and sinks (with its current cell sizes)
+// and points the descriptor at them, or clears the pointer when none are set.
+static int src_upload(bsn_grid* g) {
+    SrcDev& S = g->hs;
+    S.dx = g->dx; S.dy = g->dy; S.dz = g->dz;
+    const bool any = S.nbh > 0 || S.nsink > 0;
+    if (any) {
+        if (!g->d_src) HIPCHK(hipMalloc((void**)&g->d_src, sizeof(SrcDev)));
+        HIPCHK(hipMemcpy(g->d_src, &S, sizeof(SrcDev), hipMemcpyHostToDevice));
+    }
+    const SrcDev* want = any ? g->d_src : nullptr;
+    if (g->h.src != want) {
+        g->h.src = want;
+        CHK(grid_changed(g));
+    }
+    return BSN_OK;
+}
+
+static int fill_masses(SrcDev& S, int n, const double* pos, const double* mass, const double* eps,
+                       double x0, double y0, double z0) {
+    if (n < 0 || n > BSN_MAX_BH) return set_err(BSN_ERR_ARG, "point mass count %d (max %d)", n, BSN_MAX_BH);
+    if (n > 0 && (!pos || !mass || !eps)) return set_err(BSN_ERR_ARG, "NULL point-mass array");
+    S.nbh = n;
+    S.gx0[0] = x0; S.gx0[1] = y0; S.gx0[2] = z0;
+    for (int b = 0; b < n; ++b) {
+        S.px[b] = pos[3 * b]; S.py[b] = pos[3 * b + 1]; S.pz[b] = pos[3 * b + 2];
+        S.mass[b] = mass[b]; S.eps[b] = eps[b];
+    }
+    return BSN_OK;
+}
+
+static int fill_sinks(SrcDev& S, int n, const double* pos, const double* vel, const double* r_sink,
+                      const double* t_sink, int torque_free, double rho_sink_min, double x0,
+                      double y0, double z0) {
+    if (n < 0 || n > BSN_MAX_BH) return set_err(BSN_ERR_ARG, "sink count %d (max %d)", n, BSN_MAX_BH);
+    if (n > 0 && (!pos || !vel || !r_sink || !t_sink)) return set_err(BSN_ERR_ARG, "NULL sink array");
+    S.nsink = n;
+    S.torque_free = torque_free ? 1 : 0;
+    S.rho_sink_min = rho_sink_min;
+    S.sx0[0] = x0; S.sx0[1] = y0; S.sx0[2] = z0;
+    for (int b = 0; b < n; ++b) {
+        S.spx[b] = pos[3 * b]; S.spy[b] = pos[3 * b + 1]; S.spz[b] = pos[3 * b + 2];
+        S.svx[b] = vel[3 * b]; S.svy[b] = vel[3 * b + 1]; S.svz[b] = vel[3 * b + 2];
+        S.rs[b] = r_sink[b]; S.ts[b] = t_sink[b];
+    }
+    return BSN_OK;
+}
+
+extern "C" int bsn_grid_set_point_masses(bsn_grid* g, int n, const double* pos, const double* mass,
+                                         const double* eps, double x0, double y0, double z0) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    SrcDev S = g->hs;
+    CHK(fill_masses(S, n, pos, mass, eps, x0, y0, z0));
+    HIPCHK(hipSetDevice(g->ctx->device));
+    HIPCHK(hipStreamSynchronize(g->ctx->stream));
+    g->hs = S;
+    return src_upload(g);
+}
+
+extern "C" int bsn_grid_set_sinks(bsn_grid* g, int n, const double* pos, const double* vel,
+                                  const double* r_sink, const double* t_sink, int torque_free,
+                                  double rho_sink_min, double x0, double y0, double z0) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    SrcDev S = g->hs;
+    CHK(fill_sinks(S, n, pos, vel, r_sink, t_sink, torque_free, rho_sink_min, x0, y0, z0));
+    HIPCHK(hipSetDevice(g->ctx->device));
+    HIPCHK(hipStreamSynchronize(g->ctx->stream));
+    g->hs = S;
+    return src_upload(g);
+}
+
+// One source on a caller's dU from the grid's state, with the bodies of the
+// call (nothing is kept on the grid).
+static int add_src(bsn_grid* g, double* dU, int where, double dx, double dy, double dz,
+                   const SrcDev& S0, int which) {
+    if (!g || !dU) return set_err(BSN_ERR_ARG, "NULL argument");
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0)) return set_err(BSN_ERR_ARG, "cell sizes must be > 0");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(grid_plan(g));
+    SrcDev S = S0;
+    S.dx = dx; S.dy = dy; S.dz = dz;
+    CHK(ensure_scratch(g));
+    SrcDev* d = nullptr;
+    HIPCHK(hipMalloc((void**)&d, sizeof(SrcDev)));
+    hipError_t e = hipMemcpy(d, &S, sizeof(SrcDev), hipMemcpyHostToDevice);
+    int rc = BSN_OK;
+    if (e != hipSuccess) rc = set_err(BSN_ERR_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+    GridDev h = g->h;
+    h.src = d;
+    if (rc == BSN_OK) rc = import_ref(g, dU, where, g->d_scratch);
+    if (rc == BSN_OK) {
+        e = launch_src(g->ctx->L(), h, g->d_scratch, which);
+        if (e != hipSuccess) rc = set_err(BSN_ERR_HIP, "launch: %s", hipGetErrorString(e));
+    }
+    if (rc == BSN_OK) rc = export_ref(g, g->d_scratch, dU, where);
+    if (rc == BSN_OK) rc = sync_ctx(g->ctx);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    (void)hipFree(d);
+    return rc;
+}
+
+extern "C" int bsn_add_bh_gravity_source(bsn_grid* g, double* dU, int where, double dx, double dy,
+                                         double dz, int n, const double* pos, const double* mass,
+                                         const double* eps, double x0, double y0, double z0) {
+    SrcDev S{};
+    CHK(fill_masses(S, n, pos, mass, eps, x0, y0, z0));
+    return add_src(g, dU, where, dx, dy, dz, S, 0);
+}
+
+extern "C" int bsn_add_sink_sources(bsn_grid* g, double* dU, int where, double dx, double dy,
+                                    double dz, int n, const double* pos, const double* vel,
+                                    const double* r_sink, const double* t_sink, int torque_free,
+                                    double rho_sink_min, double x0, double y0, double z0) {
+    SrcDev S{};
+    CHK(fill_sinks(S, n, pos, vel, r_sink, t_sink, torque_free, rho_sink_min, x0, y0, z0));
+    return add_src(g, dU, where, dx, dy, dz, S, 1);
+}
+
 extern "C" int bsn_grid_set_phi(bsn_grid* g, const double* phi, int where) {
     if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
     HIPCHK(hipSetDevice(g->ctx->device));
@@ -646,7 +761,10 @@ static int apply_params(bsn_grid* g, double dx, double dy, double dz, double gam
     h.rho_floor = rho_floor;
     h.P_floor = P_floor;
     g->params_set = true;
-    if (!same) CHK(grid_changed(g));
+    if (!same) {
+        CHK(grid_changed(g));
+        if (g->hs.nbh > 0 || g->hs.nsink > 0) CHK(src_upload(g));
+    }
     return BSN_OK;
 }
 
@@ -873,11 +991,10 @@ extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    // Every kernel is timed on stage 1 of the out-of-place step: the prepared
-    // stage input lives in the work buffer C and the update writes buffer B,
-    // so the caller's state A (ghosts included) is never written.
+    // Every kernel is timed on stage 1 of the out-of-place step: it reads the
+    // caller's state A (ghosts mapped, floors on load) and the update writes
+    // buffer B, so A is never written.
     const GridDev* T1 = P.d_st[0];
-    HIPCHK(launch_prep(L, T1, P.d_poff, (int)P.grids.size(), P.ptotal));
     std::string rep = "{";
     for (int d = 0; d < 3; ++d) {
         int best = 0;
@@ -927,13 +1044,15 @@ extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             ms /= reps;
+            static const char* var[4] = {"", "pf", "w4", "ring"};
             snprintf(buf, sizeof buf, "%s\"kz%d%s\": %.4f", s ? ", " : "", zupdate_kz(s),
-                     s >= 2 ? "pf" : "", ms);
+                     var[s >> 1], ms);
             rep += buf;
             if (ms < best_ms) { best_ms = ms; best = s; }
         }
         CHK(plan_set_zshape(P, best));
-        snprintf(buf, sizeof buf, ", \"best\": \"kz%d%s\"}", zupdate_kz(best), best >= 2 ? "pf" : "");
+        static const char* vb[4] = {"", "pf", "w4", "ring"};
+        snprintf(buf, sizeof buf, ", \"best\": \"kz%d%s\"}", zupdate_kz(best), vb[best >> 1]);
         rep += buf;
     }
     rep += "}";

@@ -33,12 +33,30 @@ enum {
     SCAL_COUNT = 16
 };
 
+#define BSN_MAX_BH 8
+
+// Point masses and gas sinks of a grid (the reference's BlackHole list as
+// passed to add_bh_gravity_source! / add_sink_sources!): positions and
+// velocities, Plummer softening, sink radius and removal time per body, and
+// the left edge of the active domain each source uses for cell centres.
+struct SrcDev {
+    int nbh, nsink, torque_free, pad;
+    double dx, dy, dz;
+    double gx0[3], sx0[3];
+    double px[BSN_MAX_BH], py[BSN_MAX_BH], pz[BSN_MAX_BH], mass[BSN_MAX_BH], eps[BSN_MAX_BH];
+    double spx[BSN_MAX_BH], spy[BSN_MAX_BH], spz[BSN_MAX_BH];
+    double svx[BSN_MAX_BH], svy[BSN_MAX_BH], svz[BSN_MAX_BH];
+    double rs[BSN_MAX_BH], ts[BSN_MAX_BH];
+    double rho_sink_min;
+};
+
 struct GridDev {
     double* U;         // state the stage reads (stencils, ghost fills)
     double* Un;        // RK3 base state of the step
     double* Uout;      // where the stage update writes (== U in the in-place path)
     void* F[3];
     const double* phi;
+    const SrcDev* src;  // point masses / sinks added to every stage's RHS, or null
     double* scal;
     int64_t fs;
     int64_t ffs[3];
@@ -48,6 +66,9 @@ struct GridDev {
     int fp[3];
     int bc;            // bit a set: periodic along axis a, else outflow
     int prec;          // 0 = exact FP64, 1 = split FP32/FP64
+    int lmode;         // how a stage reads U: bit 0 = a ghost cell reads the
+                       // active cell its index maps to under bc (no ghost fill
+                       // needed); bit 1 = active cells are floored on load
     double idx, idy, idz;
     double gamma, gfac, rho_floor, P_floor;
 };

@@ -7,6 +7,8 @@
 //   _weno_fluxes_{x,y,z}_kernel!      gpu_kernels.jl:51-154   k_flux<DIR>
 //   _flux_divergence_kernel!          gpu_kernels.jl:159-169  k_update (fused)
 //   _sg_gradient_kernel!              gpu_kernels.jl:291-319  k_update (fused), k_sg_force
+//   _bh_gravity_kernel!               gpu_kernels.jl:199-230  update kernels (fused), k_src
+//   _sink_sources_kernel!             gpu_kernels.jl:238-284  update kernels (fused), k_src
 //   RK3 stage combines                euler3d.jl:244,250,256  k_update (fused), k_ghost_combine
 //   _cfl_speeds_kernel! + maximum     gpu_kernels.jl:174-191  k_update (stage 3), k_cfl
 //   _prolong_fine_ghosts!/_prolong_all!  fmr3d.jl:206-255     k_prolong
@@ -109,6 +111,8 @@ __global__ void k_ghost_fill(const GridDev* __restrict__ grids, const int64_t* _
 
 // Stage-3 combination on ghost cells, where the reference's dU is zero
 // (euler3d.jl:138 fills dU with zeros; the combine runs over the whole array).
+// With ghost mapping the stage input's ghost value is read from the active
+// cell it maps to.
 __global__ void k_ghost_combine(const GridDev* __restrict__ grids, const int64_t* __restrict__ off,
                                 int ngrids, int stage) {
     const int64_t total = off[ngrids];
@@ -119,10 +123,13 @@ __global__ void k_ghost_combine(const GridDev* __restrict__ grids, const int64_t
         int i, j, k;
         ghost_decode(G, e - off[g], i, j, k);
         const int64_t d = uidx(G, i, j, k);
+        const int64_t su = (G.lmode & 1) ? uidx(G, bc_src(i, G.nx, G.bc & 1), bc_src(j, G.ny, G.bc & 2),
+                                                bc_src(k, G.nz, G.bc & 4))
+                                         : d;
         const double dt = G.scal[SCAL_DT];
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            const double un = G.Un[q * G.fs + d], u = G.U[q * G.fs + d];
+            const double un = G.Un[q * G.fs + d], u = G.U[q * G.fs + su];
             double v;
             if (stage == 1)      v = un + dt * 0.0;
             else if (stage == 2) v = 0.75 * un + 0.25 * u + (0.25 * dt) * 0.0;
@@ -145,6 +152,26 @@ __device__ __forceinline__ bool apply_floor(double (&u)[5], double rho_floor, do
     const double P = (g - 1.0) * (u[4] - KE);
     if (P < P_floor) { u[4] = P_floor / (g - 1.0) + KE; return true; }
     return false;
+}
+
+// Loads the five values a stage sees at padded cell (i, j, k) of its input
+// U, where only axis AX may be a ghost index.  With lmode bit 0 a ghost cell
+// reads the active cell its index maps to (the value the reference's ghost
+// fill would have copied there, unfloored: the fill precedes the floors,
+// euler3d.jl:189-196); with lmode bit 1 an active cell is floored on load.
+template <int AX, bool FLOOR = true>
+__device__ __forceinline__ void load_cell(const GridDev& G, int i, int j, int k, double (&u)[5]) {
+    int si = i, sj = j, sk = k;
+    if (G.lmode & 1) {
+        if (AX == 0) si = bc_src(i, G.nx, G.bc & 1);
+        if (AX == 1) sj = bc_src(j, G.ny, G.bc & 2);
+        if (AX == 2) sk = bc_src(k, G.nz, G.bc & 4);
+    }
+    const int64_t o = uidx(G, si, sj, sk);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + o];
+    if (FLOOR && (G.lmode & 2) && si == i && sj == j && sk == k)
+        apply_floor(u, G.rho_floor, G.P_floor, G.gamma);
 }
 
 // Floors on active cells (apply_floors_3d!, euler3d.jl:73-80); with copy_un
@@ -172,35 +199,6 @@ __global__ void k_floors(const GridDev* __restrict__ grids, const int64_t* __res
 #pragma unroll
             for (int q = 0; q < 5; ++q) G.U[q * G.fs + c] = u[q];
         }
-    }
-}
-
-// Stage-1 preparation of the out-of-place step: every padded cell of the
-// stage input U is written from the raw step-start state Un -- ghost cells
-// take the raw value of the active cell their index maps to (ghost fill
-// before floors, euler3d.jl:189-196), active cells their floored raw value.
-// Un is not modified, so it stays the RK3 base state of the step.
-__global__ void k_prep(const GridDev* __restrict__ grids, const int64_t* __restrict__ off,
-                       int ngrids) {
-    const int64_t total = off[ngrids];
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int g = find_grid(off, ngrids, e);
-        const GridDev& G = grids[g];
-        const int Nx = G.nx + 2 * NG;
-        int64_t t = e - off[g];
-        const int i = (int)(t % Nx); t /= Nx;
-        const int j = (int)(t % G.Ny);
-        const int k = (int)(t / G.Ny);
-        const int si = bc_src(i, G.nx, G.bc & 1), sj = bc_src(j, G.ny, G.bc & 2),
-                  sk = bc_src(k, G.nz, G.bc & 4);
-        const int64_t s = uidx(G, si, sj, sk), d = uidx(G, i, j, k);
-        double u[5];
-#pragma unroll
-        for (int q = 0; q < 5; ++q) u[q] = G.Un[q * G.fs + s];
-        if (si == i && sj == j && sk == k) apply_floor(u, G.rho_floor, G.P_floor, G.gamma);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) G.U[q * G.fs + d] = u[q];
     }
 }
 
@@ -257,10 +255,8 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
             if (valid) {
 #pragma unroll
                 for (int m = 0; m < 6; ++m) {
-                    const int64_t o = uidx(G, G.nx + m, NG + ci, NG + cj);
                     double u[5];
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + o];
+                    load_cell<0>(G, G.nx + m, NG + ci, NG + cj, u);
                     const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) c[q][m] = (T)u[q];
@@ -284,10 +280,8 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
                     ok = tl.b + b < G.nx && pk < G.nz + 2 * NG;
                 }
                 if (ok) {
-                    const int64_t o = uidx(G, pi, pj, pk);
                     double u[5];
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + o];
+                    load_cell<DIR>(G, pi, pj, pk, u);
                     const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) lds[q][a][b] = (T)u[q];
@@ -371,17 +365,77 @@ __device__ __forceinline__ void block_max_to(uint64_t v, unsigned long long* dst
     __syncthreads();
 }
 
+// Point-mass gravity on active cell (i, j, k) (0-based) with stage-input
+// state u: d += (0, rho a, rho v.a) (src/gpu_kernels.jl:199-230).
+__device__ __forceinline__ void bh_gravity_add(const SrcDev& S, int i, int j, int k,
+                                               const double (&u)[5], double (&d)[5]) {
+    const double xc = S.gx0[0] + ((double)(i + 1) - 0.5) * S.dx;
+    const double yc = S.gx0[1] + ((double)(j + 1) - 0.5) * S.dy;
+    const double zc = S.gx0[2] + ((double)(k + 1) - 0.5) * S.dz;
+    const double r = u[0];
+    const double vx = u[1] / r, vy = u[2] / r, vz = u[3] / r;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int n = 0; n < S.nbh; ++n) {
+        const double ddx = xc - S.px[n], ddy = yc - S.py[n], ddz = zc - S.pz[n];
+        const double r2 = ddx * ddx + ddy * ddy + ddz * ddz + S.eps[n] * S.eps[n];
+        const double r3 = r2 * sqrt(r2);
+        const double fac = -S.mass[n] / r3;
+        ax += fac * ddx;
+        ay += fac * ddy;
+        az += fac * ddz;
+    }
+    d[1] += r * ax;
+    d[2] += r * ay;
+    d[3] += r * az;
+    d[4] += r * (vx * ax + vy * ay + vz * az);
+}
+
+// Gas sinks on active cell (i, j, k): torque-free or uniform drain inside
+// each sink radius (src/gpu_kernels.jl:238-284).
+__device__ __forceinline__ void sink_add(const SrcDev& S, int i, int j, int k,
+                                         const double (&u)[5], double (&d)[5]) {
+    const double r = u[0], mx = u[1], my = u[2], mz = u[3], E = u[4];
+    if (!(r > S.rho_sink_min)) return;
+    const double xc = S.sx0[0] + ((double)(i + 1) - 0.5) * S.dx;
+    const double yc = S.sx0[1] + ((double)(j + 1) - 0.5) * S.dy;
+    const double zc = S.sx0[2] + ((double)(k + 1) - 0.5) * S.dz;
+    for (int n = 0; n < S.nsink; ++n) {
+        const double ddx = xc - S.spx[n], ddy = yc - S.spy[n], ddz = zc - S.spz[n];
+        const double dist = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+        if (!(dist < S.rs[n])) continue;
+        const double t = S.ts[n];
+        d[0] -= r / t;
+        if (S.torque_free) {
+            const double rx = ddx / dist, ry = ddy / dist, rz = ddz / dist;
+            const double vx = mx / r, vy = my / r, vz = mz / r;
+            const double vrx = vx - S.svx[n], vry = vy - S.svy[n], vrz = vz - S.svz[n];
+            const double v_r = vrx * rx + vry * ry + vrz * rz;
+            const double e_int = E / r - 0.5 * (vx * vx + vy * vy + vz * vz);
+            d[1] -= r * v_r * rx / t;
+            d[2] -= r * v_r * ry / t;
+            d[3] -= r * v_r * rz / t;
+            d[4] -= (0.5 * r * (v_r * v_r) + r * e_int) / t;
+        } else {
+            d[1] -= mx / t;
+            d[2] -= my / t;
+            d[3] -= mz / t;
+            d[4] -= E / t;
+        }
+    }
+}
+
 // Phi source, stage combination and floors of active cell (i, j, k) with the
 // flux divergence d (src/gpu_kernels.jl:291-319, src/euler3d.jl:244-257);
 // the result goes to Uout.  Returns the CFL speed bits of the new state.
 __device__ __forceinline__ void cell_state(const GridDev& G, int64_t c, int stage, double (&u)[5],
                                            double (&un)[5]) {
-    const bool need_u = stage > 1 || G.phi != nullptr;
+    const bool need_u = stage > 1 || G.phi != nullptr || G.src != nullptr;
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
         un[q] = G.Un[q * G.fs + c];
         u[q] = need_u ? G.U[q * G.fs + c] : 0.0;
     }
+    if (need_u && (G.lmode & 2)) apply_floor(u, G.rho_floor, G.P_floor, G.gamma);
 }
 
 __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, int j, int k, int stage,
@@ -389,6 +443,9 @@ __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, in
                                                      const double (&un)[5], int do_cfl) {
     const int64_t c = uidx(G, NG + i, NG + j, NG + k);
     const double dt = G.scal[SCAL_DT];
+    // sources in the order of the reference's coupled step
+    // (scripts/run_sn50_fiducial.jl:772-781): point masses, self-gravity, sinks
+    if (G.src && G.src->nbh > 0) bh_gravity_add(*G.src, i, j, k, u, d);
     if (G.phi) {
         const double* ph = G.phi;
         const int ip = min(i + 1, G.nx - 1), im = max(i - 1, 0);
@@ -411,6 +468,7 @@ __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, in
         d[3] -= r * gz;
         d[4] -= r * vdotg;
     }
+    if (G.src && G.src->nsink > 0) sink_add(*G.src, i, j, k, u, d);
     double v[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -491,10 +549,19 @@ __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grid
 // finishes every cell as soon as both of its z faces are known, reading the
 // x and y fluxes from HBM.  The z fluxes never leave registers.  The stage
 // writes Uout != U, so columns can be walked in any order.
-template <typename T, int KZ, int PF>
-__global__ __launch_bounds__(256) void k_zupdate(const GridDev* __restrict__ grids,
-                                                 const Tile* __restrict__ tiles, int ntiles,
-                                                 int stage, int do_cfl) {
+//   PF:   the next window cell and the update operands are loaded before the
+//         flux arithmetic (latency overlap, more registers);
+//   WB:   minimum waves per SIMD the register allocation must allow;
+//   RING: the raw FP64 values of the last four window cells stay in a
+//         per-thread LDS ring, so the update reads the cell's own state (and,
+//         when Un is the stage input, its base state) from LDS instead of
+//         fetching it from HBM a second time.
+template <typename T, int KZ, int PF, int WB, int RING>
+__global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__ grids,
+                                                     const Tile* __restrict__ tiles, int ntiles,
+                                                     int stage, int do_cfl) {
+    __shared__ double ring[RING ? 4 : 1][5][RING ? 256 : 1];
+    const int rt = RING ? (int)threadIdx.x : 0;
     int t0, t1;
     chunk(ntiles, t0, t1);
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
@@ -512,16 +579,53 @@ __global__ __launch_bounds__(256) void k_zupdate(const GridDev* __restrict__ gri
             const int64_t col = uidx(G, NG + i, NG + j, 0);
             const int64_t zs = (int64_t)G.Ny * G.pitch;
             double nxt[5];
+            int npz = 0;
             auto fetch = [&](int pz) {
-                const int64_t o = col + (int64_t)pz * zs;
-#pragma unroll
-                for (int q = 0; q < 5; ++q) nxt[q] = G.U[q * G.fs + o];
+                npz = pz;
+                load_cell<2, !RING>(G, NG + i, NG + j, pz, nxt);
             };
             auto place = [&](int slot) {
+                if (RING) {
+                    // raw value (unfloored) into the ring; slot npz & 3 is not
+                    // read again before cell npz + 4 replaces it
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) ring[npz & 3][q][rt] = nxt[q];
+                    if ((G.lmode & 2) && npz >= NG && npz < NG + G.nz)
+                        apply_floor(nxt, G.rho_floor, G.P_floor, G.gamma);
+                }
                 const double pr = ex::cell_pressure(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], G.gamma);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) c[q][slot] = (T)nxt[q];
                 p[slot] = (T)pr;
+            };
+            T fx[2][5], fy[2][5];
+            double u[5], un[5];
+            // the operands of the update of active cell k
+            auto operands = [&](int k) {
+                const int64_t ox = ((int64_t)k * G.ny + j) * G.fp[0] + i;
+                const int64_t oy = ((int64_t)k * (G.ny + 1) + j) * G.fp[1] + i;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    fx[0][q] = Fx[q * G.ffs[0] + ox];
+                    fx[1][q] = Fx[q * G.ffs[0] + ox + 1];
+                    fy[0][q] = Fy[q * G.ffs[1] + oy];
+                    fy[1][q] = Fy[q * G.ffs[1] + oy + G.fp[1]];
+                }
+                const int64_t cc = col + (int64_t)(NG + k) * zs;
+                if (RING) {
+                    const int sl = (NG + k) & 3;
+                    const bool un_in_ring = G.Un == G.U;
+                    const bool need_u = stage > 1 || G.phi != nullptr || G.src != nullptr;
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) {
+                        const double r = ring[sl][q][rt];
+                        u[q] = r;
+                        un[q] = un_in_ring ? r : G.Un[q * G.fs + cc];
+                    }
+                    if (need_u && (G.lmode & 2)) apply_floor(u, G.rho_floor, G.P_floor, G.gamma);
+                } else {
+                    cell_state(G, cc, stage, u, un);
+                }
             };
 #pragma unroll
             for (int m = 0; m < 5; ++m) { fetch(k0 + m); place(m); }
@@ -530,38 +634,12 @@ __global__ __launch_bounds__(256) void k_zupdate(const GridDev* __restrict__ gri
             for (int f = k0; f <= kend; ++f) {
                 place(5);                               // cell f+5
                 if (PF && f < kend) fetch(f + 6);       // next iteration's cell, in flight during the flux
-                // operands of the update of cell k = f-1; with PF they are
-                // issued before the flux arithmetic so their latency overlaps it
-                const int k = f - 1;
-                T fx[2][5], fy[2][5];
-                double u[5], un[5];
-                if (PF && f > k0) {
-                    const int64_t ox = ((int64_t)k * G.ny + j) * G.fp[0] + i;
-                    const int64_t oy = ((int64_t)k * (G.ny + 1) + j) * G.fp[1] + i;
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) {
-                        fx[0][q] = Fx[q * G.ffs[0] + ox];
-                        fx[1][q] = Fx[q * G.ffs[0] + ox + 1];
-                        fy[0][q] = Fy[q * G.ffs[1] + oy];
-                        fy[1][q] = Fy[q * G.ffs[1] + oy + G.fp[1]];
-                    }
-                    cell_state(G, col + (int64_t)(NG + k) * zs, stage, u, un);
-                }
+                const int k = f - 1;                    // the cell finished in this iteration
+                if (PF && f > k0) operands(k);
                 T F[5];
                 Prec<T>::template flux<2>(c, p, G, F);
                 if (!PF && f < kend) fetch(f + 6);
-                if (!PF && f > k0) {
-                    const int64_t ox = ((int64_t)k * G.ny + j) * G.fp[0] + i;
-                    const int64_t oy = ((int64_t)k * (G.ny + 1) + j) * G.fp[1] + i;
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) {
-                        fx[0][q] = Fx[q * G.ffs[0] + ox];
-                        fx[1][q] = Fx[q * G.ffs[0] + ox + 1];
-                        fy[0][q] = Fy[q * G.ffs[1] + oy];
-                        fy[1][q] = Fy[q * G.ffs[1] + oy + G.fp[1]];
-                    }
-                    cell_state(G, col + (int64_t)(NG + k) * zs, stage, u, un);
-                }
+                if (!PF && f > k0) operands(k);
                 if (f > k0) {
                     double d[5];
 #pragma unroll
@@ -736,6 +814,30 @@ __global__ void k_sg_force(GridDev G, double* __restrict__ dU) {
         dU[2 * G.fs + c] -= rr * gy;
         dU[3 * G.fs + c] -= rr * gz;
         dU[4 * G.fs + c] -= rr * vdotg;
+    }
+}
+
+// Point-mass gravity (which = 0) or sink (which = 1) source on a dU buffer
+// with U's layout, from the grid's state (add_bh_gravity_source!,
+// src/gravity_bh.jl:58-78; add_sink_sources!, src/sinks.jl:32-58).
+__global__ void k_src(GridDev G, double* __restrict__ dU, int which) {
+    const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % G.nx);
+        const int64_t r = e / G.nx;
+        const int j = (int)(r % G.ny), k = (int)(r / G.ny);
+        const int64_t c = uidx(G, NG + i, NG + j, NG + k);
+        double u[5], d[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            u[q] = G.U[q * G.fs + c];
+            d[q] = dU[q * G.fs + c];
+        }
+        if (which == 0) bh_gravity_add(*G.src, i, j, k, u, d);
+        else            sink_add(*G.src, i, j, k, u, d);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) dU[q * G.fs + c] = d[q];
     }
 }
 
@@ -1122,28 +1224,36 @@ hipError_t launch_update(const LaunchCtx& L, int prec, const GridDev* grids, con
 
 static int flat_blocks(int64_t n, int cus);
 
-template <typename T, int KZ, int PF>
+template <typename T, int KZ, int PF, int WB, int RING>
 static hipError_t launch_zupdate_t(const LaunchCtx& L, const GridDev* grids, const Tile* tiles,
                                    int ntiles, int stage, int do_cfl) {
-    const void* fn = (const void*)&k_zupdate<T, KZ, PF>;
+    const void* fn = (const void*)&k_zupdate<T, KZ, PF, WB, RING>;
     const int g = occupancy_grid(fn, 256, 0, ntiles, L.cus);
-    hipLaunchKernelGGL((k_zupdate<T, KZ, PF>), dim3(g), dim3(256), 0, L.stream, grids, tiles,
-                       ntiles, stage, do_cfl);
+    hipLaunchKernelGGL((k_zupdate<T, KZ, PF, WB, RING>), dim3(g), dim3(256), 0, L.stream, grids,
+                       tiles, ntiles, stage, do_cfl);
     return hipGetLastError();
 }
 
-// shapes: 0 kz16, 1 kz32, 2 kz16 with prefetch, 3 kz32 with prefetch
+// Shapes (bit 0: 16 or 32 cells per thread along z; bits 1-2: variant):
+// variant 0 plain, 1 operands loaded one cell ahead, 2 register budget
+// capped for 4 waves per SIMD (FP32 flux only; the FP64 form keeps variant
+// 0), 3 LDS ring of the window's raw cells.
 int zupdate_kz(int shape) { return (shape & 1) ? 32 : 16; }
-int zupdate_shape_count() { return 4; }
+int zupdate_shape_count() { return 8; }
 
 template <typename T>
 static hipError_t launch_zupdate_p(const LaunchCtx& L, int shape, const GridDev* grids,
                                    const Tile* tiles, int ntiles, int stage, int do_cfl) {
+    constexpr bool F = sizeof(T) == 4;
     switch (shape) {
-        case 1: return launch_zupdate_t<T, 32, 0>(L, grids, tiles, ntiles, stage, do_cfl);
-        case 2: return launch_zupdate_t<T, 16, 1>(L, grids, tiles, ntiles, stage, do_cfl);
-        case 3: return launch_zupdate_t<T, 32, 1>(L, grids, tiles, ntiles, stage, do_cfl);
-        default: return launch_zupdate_t<T, 16, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 1: return launch_zupdate_t<T, 32, 0, 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 2: return launch_zupdate_t<T, 16, 1, F ? 3 : 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 3: return launch_zupdate_t<T, 32, 1, F ? 3 : 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 4: return launch_zupdate_t<T, 16, 0, F ? 4 : 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 5: return launch_zupdate_t<T, 32, 0, F ? 4 : 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 6: return launch_zupdate_t<T, 16, 0, 1, 1>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 7: return launch_zupdate_t<T, 32, 0, 1, 1>(L, grids, tiles, ntiles, stage, do_cfl);
+        default: return launch_zupdate_t<T, 16, 0, 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
     }
 }
 
@@ -1152,14 +1262,6 @@ hipError_t launch_zupdate(const LaunchCtx& L, int prec, int shape, const GridDev
     if (ntiles <= 0) return hipSuccess;
     if (prec == 0) return launch_zupdate_p<double>(L, shape, grids, tiles, ntiles, stage, do_cfl);
     return launch_zupdate_p<float>(L, shape, grids, tiles, ntiles, stage, do_cfl);
-}
-
-hipError_t launch_prep(const LaunchCtx& L, const GridDev* grids, const int64_t* off, int ngrids,
-                       int64_t total) {
-    if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_prep, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids, off,
-                       ngrids);
-    return hipGetLastError();
 }
 
 hipError_t launch_cfl(const LaunchCtx& L, const GridDev* grids, const Tile* tiles, int ntiles) {
@@ -1246,6 +1348,12 @@ hipError_t launch_flux_export(const LaunchCtx& L, const GridDev& G, int dir, dou
 hipError_t launch_sg_force(const LaunchCtx& L, const GridDev& G, double* dU) {
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
     hipLaunchKernelGGL(k_sg_force, dim3(flat_blocks(n, L.cus)), dim3(256), 0, L.stream, G, dU);
+    return hipGetLastError();
+}
+
+hipError_t launch_src(const LaunchCtx& L, const GridDev& G, double* dU, int which) {
+    const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    hipLaunchKernelGGL(k_src, dim3(flat_blocks(n, L.cus)), dim3(256), 0, L.stream, G, dU, which);
     return hipGetLastError();
 }
 

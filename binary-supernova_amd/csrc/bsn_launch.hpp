@@ -24,8 +24,6 @@ int zupdate_kz(int shape);
 int zupdate_shape_count();
 hipError_t launch_zupdate(const LaunchCtx& L, int prec, int shape, const GridDev* grids,
                           const Tile* tiles, int ntiles, int stage, int do_cfl);
-hipError_t launch_prep(const LaunchCtx& L, const GridDev* grids, const int64_t* off, int ngrids,
-                       int64_t total);
 hipError_t launch_ghost_fill(const LaunchCtx& L, const GridDev* grids, const int64_t* off,
                              int ngrids, int64_t total, int copy_un);
 hipError_t launch_ghost_combine(const LaunchCtx& L, const GridDev* grids, const int64_t* off,
@@ -40,6 +38,7 @@ hipError_t launch_aos_to_soa(const LaunchCtx& L, const GridDev& G, const double*
 hipError_t launch_soa_to_aos(const LaunchCtx& L, const GridDev& G, const double* src, double* aos);
 hipError_t launch_flux_export(const LaunchCtx& L, const GridDev& G, int dir, double* out);
 hipError_t launch_sg_force(const LaunchCtx& L, const GridDev& G, double* dU);
+hipError_t launch_src(const LaunchCtx& L, const GridDev& G, double* dU, int which);
 hipError_t launch_combine_all(const LaunchCtx& L, const GridDev& G, int stage, const double* dU,
                               double dt);
 hipError_t launch_copy_state(const LaunchCtx& L, const GridDev& G, const double* src, double* dst);

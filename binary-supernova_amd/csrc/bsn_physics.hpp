@@ -290,28 +290,31 @@ __device__ __forceinline__ void weno5_pair(float a0, float a1, float a2, float a
 
 // The reference's HLLC (src/hllc.jl:17-165) in FP32: same branches and
 // guards; 1e-100 thresholds become 1e-37 (the FP32 normal range).
+// m2X, irX and KE_X are |m|^2, 1/max(rho, 1e-30) and the capped kinetic
+// energy of each side as the interface energy rebuild formed them; they are
+// formed again here only where a guard changes the state.
 __device__ __forceinline__ void hllc_normal(float rL, float mnL, float mt1L, float mt2L, float EL,
+                                            float m2L, float irL, float KE_L,
                                             float rR, float mnR, float mt1R, float mt2R, float ER,
+                                            float m2R, float irR, float KE_R,
                                             float g, float gfac, float F[5]) {
-    if (rL <= 0.0f) { rL = 1e-30f; mnL = 0.0f; mt1L = 0.0f; mt2L = 0.0f; EL = 0.0f; }
+    // rho <= 0 resets the side; 1/max(rho, 1e-30) is then already 1e30
+    if (rL <= 0.0f) { rL = 1e-30f; mnL = 0.0f; mt1L = 0.0f; mt2L = 0.0f; EL = 0.0f; m2L = 0.0f; KE_L = 0.0f; }
     else            { rL = fmaxj(rL, 1e-30f); }
-    if (rR <= 0.0f) { rR = 1e-30f; mnR = 0.0f; mt1R = 0.0f; mt2R = 0.0f; ER = 0.0f; }
+    if (rR <= 0.0f) { rR = 1e-30f; mnR = 0.0f; mt1R = 0.0f; mt2R = 0.0f; ER = 0.0f; m2R = 0.0f; KE_R = 0.0f; }
     else            { rR = fmaxj(rR, 1e-30f); }
-    float m2L = mnL * mnL + mt1L * mt1L + mt2L * mt2L;
     if (m2L > (rL * rL) * 1e4f) {
         float s = (rL * 100.0f) * __builtin_amdgcn_rsqf(m2L);
         mnL *= s; mt1L *= s; mt2L *= s;
         m2L = mnL * mnL + mt1L * mt1L + mt2L * mt2L;
+        KE_L = fminj(0.5f * m2L * irL, 0.5f * rL * 1e4f);
     }
-    float m2R = mnR * mnR + mt1R * mt1R + mt2R * mt2R;
     if (m2R > (rR * rR) * 1e4f) {
         float s = (rR * 100.0f) * __builtin_amdgcn_rsqf(m2R);
         mnR *= s; mt1R *= s; mt2R *= s;
         m2R = mnR * mnR + mt1R * mt1R + mt2R * mt2R;
+        KE_R = fminj(0.5f * m2R * irR, 0.5f * rR * 1e4f);
     }
-    float irL = rcp(rL), irR = rcp(rR);
-    float KE_L = fminj(0.5f * m2L * irL, 0.5f * rL * 1e4f);
-    float KE_R = fminj(0.5f * m2R * irR, 0.5f * rR * 1e4f);
     EL = fmaxj(EL, KE_L);
     ER = fmaxj(ER, KE_R);
     float vnL = mnL * irL, vt1L = mt1L * irL, vt2L = mt2L * irL;
@@ -384,19 +387,25 @@ __device__ __forceinline__ void interface_flux(const float (&c)[4][6], const flo
     float PL = fminj(fmaxj(PLw, 0.0f), P_maxL);
     float PR = fminj(fmaxj(PRw, 0.0f), P_maxR);
     float rpL = fmaxj(L[0], 1e-30f), rpR = fmaxj(R[0], 1e-30f);
-    float KEL = fminj(0.5f * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]) * rcp(rpL), 0.5f * rpL * 1e4f);
-    float KER = fminj(0.5f * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]) * rcp(rpR), 0.5f * rpR * 1e4f);
+    float irL = rcp(rpL), irR = rcp(rpR);
+    float m2L = L[1] * L[1] + L[2] * L[2] + L[3] * L[3];
+    float m2R = R[1] * R[1] + R[2] * R[2] + R[3] * R[3];
+    float KEL = fminj(0.5f * m2L * irL, 0.5f * rpL * 1e4f);
+    float KER = fminj(0.5f * m2R * irR, 0.5f * rpR * 1e4f);
     float EL = PL * ig1 + KEL;
     float ER = PR * ig1 + KER;
     float G[5];
     if (DIR == 0) {
-        hllc_normal(L[0], L[1], L[2], L[3], EL, R[0], R[1], R[2], R[3], ER, g, gfac, G);
+        hllc_normal(L[0], L[1], L[2], L[3], EL, m2L, irL, KEL, R[0], R[1], R[2], R[3], ER, m2R, irR, KER,
+                    g, gfac, G);
         F[0] = G[0]; F[1] = G[1]; F[2] = G[2]; F[3] = G[3]; F[4] = G[4];
     } else if (DIR == 1) {
-        hllc_normal(L[0], L[2], L[1], L[3], EL, R[0], R[2], R[1], R[3], ER, g, gfac, G);
+        hllc_normal(L[0], L[2], L[1], L[3], EL, m2L, irL, KEL, R[0], R[2], R[1], R[3], ER, m2R, irR, KER,
+                    g, gfac, G);
         F[0] = G[0]; F[1] = G[2]; F[2] = G[1]; F[3] = G[3]; F[4] = G[4];
     } else {
-        hllc_normal(L[0], L[3], L[1], L[2], EL, R[0], R[3], R[1], R[2], ER, g, gfac, G);
+        hllc_normal(L[0], L[3], L[1], L[2], EL, m2L, irL, KEL, R[0], R[3], R[1], R[2], ER, m2R, irR, KER,
+                    g, gfac, G);
         F[0] = G[0]; F[1] = G[2]; F[2] = G[3]; F[3] = G[1]; F[4] = G[4];
     }
 }

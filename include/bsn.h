@@ -106,6 +106,35 @@ int bsn_rk3_combine(bsn_grid* g, int stage, const double* dU, int where, double 
 /* Conservation totals sum(U[q, active]) * dV, q = 1..5 (src/diagnostics.jl:21-58). */
 int bsn_conserved_totals(bsn_grid* g, double dV, double out[5]);
 
+/* Point masses and gas sinks ---------------------------------------------- */
+/* The bodies of the reference's BlackHole list (src/BinarySupernova.jl:130-137)
+ * as plain arrays: pos and vel hold 3 values per body (x, y, z); n <= 8.
+ * x0, y0, z0 are the left edge of the active domain, so active cell ii
+ * (1-based) has centre x0 + (ii - 0.5)*dx.  Sink radius and removal time are
+ * the caller's r_sink(bh) and t_sink(bh, f_sink) (src/BinarySupernova.jl:159-171). */
+#define BSN_MAX_BH 8
+/* add_bh_gravity_source!(dU, U, nx, ny, nz, dx, dy, dz, bhs, x0, y0, z0)
+ * (src/gravity_bh.jl:58-78): adds point-mass gravity from the grid's state to
+ * dU (reference layout). */
+int bsn_add_bh_gravity_source(bsn_grid* g, double* dU, int where, double dx, double dy, double dz,
+                              int n, const double* pos, const double* mass, const double* eps,
+                              double x0, double y0, double z0);
+/* add_sink_sources!(dU, U, ...; f_sink, torque_free, rho_sink_min)
+ * (src/sinks.jl:32-58): adds the sink drains of the n bodies to dU. */
+int bsn_add_sink_sources(bsn_grid* g, double* dU, int where, double dx, double dy, double dz,
+                         int n, const double* pos, const double* vel, const double* r_sink,
+                         const double* t_sink, int torque_free, double rho_sink_min, double x0,
+                         double y0, double z0);
+/* Sources every later step of the grid adds to each stage's RHS, in the order
+ * of the reference's coupled step (scripts/run_sn50_fiducial.jl:772-816):
+ * flux divergence, point-mass gravity, the phi source, sinks.  n = 0 removes
+ * them.  The step uses the grid's cell sizes. */
+int bsn_grid_set_point_masses(bsn_grid* g, int n, const double* pos, const double* mass,
+                              const double* eps, double x0, double y0, double z0);
+int bsn_grid_set_sinks(bsn_grid* g, int n, const double* pos, const double* vel,
+                       const double* r_sink, const double* t_sink, int torque_free,
+                       double rho_sink_min, double x0, double y0, double z0);
+
 /* Host-array convenience drop-ins: the reference signature on a host Julia
  * Array U; a cached per-thread grid uploads, runs and downloads. */
 int bsn_euler3d_step_host(double* U, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
@@ -124,9 +153,11 @@ int bsn_batch_destroy(bsn_batch* b);
 int bsn_batch_autotune(bsn_batch* b, char* report, int report_len);
 /* Sets the flux tile shape ids explicitly (x: 0..2, y/z: 0..3). */
 int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz);
-/* Sets the shape id of the fused z-sweep + stage-update kernel used by the
- * out-of-place step: 0 = 16 cells per thread along z, 1 = 32, 2 and 3 = the
- * same with the next cell and the update operands loaded one cell ahead. */
+/* Sets the shape id (0..7) of the fused z-sweep + stage-update kernel used by
+ * the out-of-place step: even = 16 cells per thread along z, odd = 32; 2 and 3
+ * load the next cell and the update operands one cell ahead; 4 and 5 cap the
+ * register budget at 4 waves per SIMD (split precision); 6 and 7 keep the
+ * window's raw cells in an LDS ring instead of reading them twice. */
 int bsn_batch_set_update_shape(bsn_batch* b, int shape);
 /* Step form: 0 (default) = out of place, z sweep fused with the stage update;
  * 1 = in place with all fluxes materialised.  Both give the same bits. */

@@ -19,6 +19,9 @@
  *   src/fmr3d.jl:32-622       two-level FMR (prolong, restrict, flux registers,
  *                             Berger-Colella correction, sub-cycled step, CFL)
  *   src/diagnostics.jl:21-58  conservation totals (summation order not pinned)
+ *   src/gpu_kernels.jl:199-284 point-mass gravity and gas-sink sources, and
+ *                             their composition in coupled_step!
+ *                             (scripts/run_sn50_fiducial.jl:772-816)
  *
  * Layout is the reference's: U is (5, Nx, Ny, Nz), column-major with the field
  * index q fastest, N? = n? + 2*NG, NG = 3.  Indices below are 0-based: the
@@ -541,6 +544,144 @@ int orc_euler3d_step(double *U, int64_t nx, int64_t ny, int64_t nz,
         orc_euler3d_rhs(dU, Fx, Fy, Fz, U, nx, ny, nz, dx, dy, dz, g, bc,
                         rho_floor, P_floor, 1);
         if (phi) orc_apply_sg_force(dU, U, phi, nx, ny, nz, dx, dy, dz);
+        orc_rk3_combine(stage, U, Un, dU, dt, n);
+        orc_apply_floors(U, nx, ny, nz, rho_floor, P_floor, g);
+    }
+    free(Fx); free(Fy); free(Fz); free(dU); free(Un);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Point-mass gravity and gas sinks (src/gpu_kernels.jl:199-284, driven by
+ * add_bh_gravity_source! src/gravity_bh.jl:58-78 and add_sink_sources!
+ * src/sinks.jl:32-58).  Cell centres are x0 + (ii - 0.5)*dx for the 1-based
+ * active index ii.  The sources read U (the stage input after its floors).  */
+
+void orc_bh_gravity_source(double *dU, const double *U, int64_t nx, int64_t ny, int64_t nz,
+                           double dx, double dy, double dz, double x0, double y0, double z0,
+                           int nbh, const double *px, const double *py, const double *pz,
+                           const double *mass, const double *eps)
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG;
+#pragma omp parallel for schedule(static)
+    for (int64_t kk = 1; kk <= nz; ++kk)
+        for (int64_t jj = 1; jj <= ny; ++jj)
+            for (int64_t ii = 1; ii <= nx; ++ii) {
+                const int64_t i = ii + NG - 1, j = jj + NG - 1, k = kk + NG - 1;
+                const double xc = x0 + ((double)ii - 0.5) * dx;
+                const double yc = y0 + ((double)jj - 0.5) * dy;
+                const double zc = z0 + ((double)kk - 0.5) * dz;
+                const double r = U[U_IDX(0, i, j, k, Nx, Ny)];
+                const double vx = U[U_IDX(1, i, j, k, Nx, Ny)] / r;
+                const double vy = U[U_IDX(2, i, j, k, Nx, Ny)] / r;
+                const double vz = U[U_IDX(3, i, j, k, Nx, Ny)] / r;
+                double ax = 0.0, ay = 0.0, az = 0.0;
+                for (int n = 0; n < nbh; ++n) {
+                    const double ddx = xc - px[n], ddy = yc - py[n], ddz = zc - pz[n];
+                    const double r2 = ddx * ddx + ddy * ddy + ddz * ddz + eps[n] * eps[n];
+                    const double r3 = r2 * sqrt(r2);
+                    const double fac = -mass[n] / r3;
+                    ax += fac * ddx;
+                    ay += fac * ddy;
+                    az += fac * ddz;
+                }
+                dU[U_IDX(1, i, j, k, Nx, Ny)] += r * ax;
+                dU[U_IDX(2, i, j, k, Nx, Ny)] += r * ay;
+                dU[U_IDX(3, i, j, k, Nx, Ny)] += r * az;
+                dU[U_IDX(4, i, j, k, Nx, Ny)] += r * (vx * ax + vy * ay + vz * az);
+            }
+}
+
+void orc_sink_sources(double *dU, const double *U, int64_t nx, int64_t ny, int64_t nz,
+                      double dx, double dy, double dz, double x0, double y0, double z0,
+                      int nbh, const double *px, const double *py, const double *pz,
+                      const double *bvx, const double *bvy, const double *bvz,
+                      const double *rs, const double *ts, int torque_free, double rho_sink_min)
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG;
+#pragma omp parallel for schedule(static)
+    for (int64_t kk = 1; kk <= nz; ++kk)
+        for (int64_t jj = 1; jj <= ny; ++jj)
+            for (int64_t ii = 1; ii <= nx; ++ii) {
+                const int64_t i = ii + NG - 1, j = jj + NG - 1, k = kk + NG - 1;
+                const double xc = x0 + ((double)ii - 0.5) * dx;
+                const double yc = y0 + ((double)jj - 0.5) * dy;
+                const double zc = z0 + ((double)kk - 0.5) * dz;
+                const double r = U[U_IDX(0, i, j, k, Nx, Ny)];
+                const double mx = U[U_IDX(1, i, j, k, Nx, Ny)];
+                const double my = U[U_IDX(2, i, j, k, Nx, Ny)];
+                const double mz = U[U_IDX(3, i, j, k, Nx, Ny)];
+                const double E = U[U_IDX(4, i, j, k, Nx, Ny)];
+                if (!(r > rho_sink_min)) continue;
+                for (int n = 0; n < nbh; ++n) {
+                    const double ddx = xc - px[n], ddy = yc - py[n], ddz = zc - pz[n];
+                    const double d = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                    if (!(d < rs[n])) continue;
+                    const double t = ts[n];
+                    dU[U_IDX(0, i, j, k, Nx, Ny)] -= r / t;
+                    if (torque_free) {
+                        const double rx = ddx / d, ry = ddy / d, rz = ddz / d;
+                        const double vx = mx / r, vy = my / r, vz = mz / r;
+                        const double vrx = vx - bvx[n], vry = vy - bvy[n], vrz = vz - bvz[n];
+                        const double v_r = vrx * rx + vry * ry + vrz * rz;
+                        const double e_int = E / r - 0.5 * (vx * vx + vy * vy + vz * vz);
+                        dU[U_IDX(1, i, j, k, Nx, Ny)] -= r * v_r * rx / t;
+                        dU[U_IDX(2, i, j, k, Nx, Ny)] -= r * v_r * ry / t;
+                        dU[U_IDX(3, i, j, k, Nx, Ny)] -= r * v_r * rz / t;
+                        dU[U_IDX(4, i, j, k, Nx, Ny)] -= (0.5 * r * (v_r * v_r) + r * e_int) / t;
+                    } else {
+                        dU[U_IDX(1, i, j, k, Nx, Ny)] -= mx / t;
+                        dU[U_IDX(2, i, j, k, Nx, Ny)] -= my / t;
+                        dU[U_IDX(3, i, j, k, Nx, Ny)] -= mz / t;
+                        dU[U_IDX(4, i, j, k, Nx, Ny)] -= E / t;
+                    }
+                }
+            }
+}
+
+/* The hydrodynamic part of the reference's coupled SSP-RK3 step
+ * (coupled_step!, scripts/run_sn50_fiducial.jl:772-816): per stage the RHS,
+ * then point-mass gravity from every BH, the self-gravity source for a given
+ * potential phi (if any; the reference solves it per stage), the sink drains
+ * of the first nsink BHs, the stage combination and the floors.  The N-body
+ * update and accrete! stay with the caller.  The bc is the step's argument
+ * (the driver uses outflow).                                                 */
+int orc_coupled_step(double *U, int64_t nx, int64_t ny, int64_t nz,
+                     double dx, double dy, double dz, double dt, double g,
+                     const int bc[3], double rho_floor, double P_floor, const double *phi,
+                     double x0, double y0, double z0, int nbh, const double *pos,
+                     const double *vel, const double *mass, const double *eps,
+                     int nsink, const double *rs, const double *ts, int torque_free,
+                     double rho_sink_min)
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG, Nz = nz + 2 * NG;
+    const int64_t n = 5 * Nx * Ny * Nz;
+    double px[16], py[16], pz[16], vx[16], vy[16], vz[16];
+    if (nbh > 16 || nsink > nbh) return -2;
+    for (int b = 0; b < nbh; ++b) {
+        px[b] = pos[3 * b]; py[b] = pos[3 * b + 1]; pz[b] = pos[3 * b + 2];
+        vx[b] = vel[3 * b]; vy[b] = vel[3 * b + 1]; vz[b] = vel[3 * b + 2];
+    }
+    double *Fx = malloc(sizeof(double) * 5 * (Nx + 1) * Ny * Nz);
+    double *Fy = malloc(sizeof(double) * 5 * Nx * (Ny + 1) * Nz);
+    double *Fz = malloc(sizeof(double) * 5 * Nx * Ny * (Nz + 1));
+    double *dU = malloc(sizeof(double) * n);
+    double *Un = malloc(sizeof(double) * n);
+    if (!Fx || !Fy || !Fz || !dU || !Un) {
+        free(Fx); free(Fy); free(Fz); free(dU); free(Un);
+        return -1;
+    }
+    memcpy(Un, U, sizeof(double) * n);
+    for (int stage = 1; stage <= 3; ++stage) {
+        orc_euler3d_rhs(dU, Fx, Fy, Fz, U, nx, ny, nz, dx, dy, dz, g, bc,
+                        rho_floor, P_floor, 1);
+        if (nbh > 0)
+            orc_bh_gravity_source(dU, U, nx, ny, nz, dx, dy, dz, x0, y0, z0, nbh, px, py, pz,
+                                  mass, eps);
+        if (phi) orc_apply_sg_force(dU, U, phi, nx, ny, nz, dx, dy, dz);
+        if (nsink > 0)
+            orc_sink_sources(dU, U, nx, ny, nz, dx, dy, dz, x0, y0, z0, nsink, px, py, pz,
+                             vx, vy, vz, rs, ts, torque_free, rho_sink_min);
         orc_rk3_combine(stage, U, Un, dU, dt, n);
         orc_apply_floors(U, nx, ny, nz, rho_floor, P_floor, g);
     }

@@ -60,6 +60,22 @@ int orc_fmr_multilevel_step(int nlev, double **U, const int64_t *dims, const dou
                             const int64_t *lo_hi, const int *ratio, double g, const int bc[3],
                             double rho_floor, double P_floor, double dt,
                             const double *const *phis);
+void orc_bh_gravity_source(double *dU, const double *U, int64_t nx, int64_t ny, int64_t nz,
+                           double dx, double dy, double dz, double x0, double y0, double z0,
+                           int nbh, const double *px, const double *py, const double *pz,
+                           const double *mass, const double *eps);
+void orc_sink_sources(double *dU, const double *U, int64_t nx, int64_t ny, int64_t nz,
+                      double dx, double dy, double dz, double x0, double y0, double z0,
+                      int nbh, const double *px, const double *py, const double *pz,
+                      const double *bvx, const double *bvy, const double *bvz,
+                      const double *rs, const double *ts, int torque_free, double rho_sink_min);
+int orc_coupled_step(double *U, int64_t nx, int64_t ny, int64_t nz,
+                     double dx, double dy, double dz, double dt, double g,
+                     const int bc[3], double rho_floor, double P_floor, const double *phi,
+                     double x0, double y0, double z0, int nbh, const double *pos,
+                     const double *vel, const double *mass, const double *eps,
+                     int nsink, const double *rs, const double *ts, int torque_free,
+                     double rho_sink_min);
 
 #ifdef __cplusplus
 }

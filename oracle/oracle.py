@@ -83,6 +83,14 @@ def _declare(L):
     L.orc_fmr_multilevel_step.argtypes = [C.c_int, C.POINTER(_pd), _pi64, _pd, _pi64, _pi,
                                           _d, _pi, _d, _d, _d, C.POINTER(_pd)]
     L.orc_fmr_multilevel_step.restype = C.c_int
+    L.orc_bh_gravity_source.argtypes = [_pd, _pd, _i64, _i64, _i64, _d, _d, _d, _d, _d, _d,
+                                        C.c_int, _pd, _pd, _pd, _pd, _pd]
+    L.orc_sink_sources.argtypes = [_pd, _pd, _i64, _i64, _i64, _d, _d, _d, _d, _d, _d, C.c_int,
+                                   _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, C.c_int, _d]
+    L.orc_coupled_step.argtypes = [_pd, _i64, _i64, _i64, _d, _d, _d, _d, _d, _pi, _d, _d, _pd,
+                                   _d, _d, _d, C.c_int, _pd, _pd, _pd, _pd, C.c_int, _pd, _pd,
+                                   C.c_int, _d]
+    L.orc_coupled_step.restype = C.c_int
 
 
 def _p(a):
@@ -258,3 +266,48 @@ def fmr_multilevel_step(Us, dxs, lo_his, ratios, gamma, dt, bc="outflow", rho_fl
                                        rho_floor, P_floor, dt, ph)
     if rc:
         raise MemoryError("oracle multilevel step failed")
+
+
+# -- point masses and sinks (src/gpu_kernels.jl:199-284) ---------------------
+
+def _v(vals):
+    a = np.ascontiguousarray(np.asarray(vals, dtype=np.float64).ravel())
+    return a
+
+
+def bh_gravity_source(dU, U, dx, dy, dz, x0, y0, z0, pos, mass, eps):
+    """_bh_gravity_kernel! over the active cells; pos is (n, 3)."""
+    nx, ny, nz = _dims(U)
+    p = _v(pos).reshape(-1, 3)
+    px, py, pz = (np.ascontiguousarray(p[:, a]) for a in range(3))
+    m, e = _v(mass), _v(eps)
+    lib().orc_bh_gravity_source(_p(dU), _p(U), nx, ny, nz, dx, dy, dz, x0, y0, z0, len(m),
+                                _p(px), _p(py), _p(pz), _p(m), _p(e))
+
+
+def sink_sources(dU, U, dx, dy, dz, x0, y0, z0, pos, vel, rs, ts, torque_free=True,
+                 rho_sink_min=0.0):
+    """_sink_sources_kernel! over the active cells; pos, vel are (n, 3)."""
+    nx, ny, nz = _dims(U)
+    p = _v(pos).reshape(-1, 3)
+    v = _v(vel).reshape(-1, 3)
+    cols = [np.ascontiguousarray(a[:, c]) for a in (p, v) for c in range(3)]
+    r, t = _v(rs), _v(ts)
+    lib().orc_sink_sources(_p(dU), _p(U), nx, ny, nz, dx, dy, dz, x0, y0, z0, len(r),
+                           *[_p(c) for c in cols], _p(r), _p(t), int(bool(torque_free)),
+                           rho_sink_min)
+
+
+def coupled_step(U, dx, dy, dz, dt, gamma, bc="outflow", rho_floor=0.0, P_floor=0.0, phi=None,
+                 origin=(0.0, 0.0, 0.0), pos=(), vel=(), mass=(), eps=(), nsink=0, rs=(), ts=(),
+                 torque_free=True, rho_sink_min=0.0):
+    """The hydrodynamic part of coupled_step! (scripts/run_sn50_fiducial.jl:
+    772-816): gravity from every body, sinks from the first nsink."""
+    nx, ny, nz = _dims(U)
+    p, v, m, e = _v(pos), _v(vel), _v(mass), _v(eps)
+    r, t = _v(rs), _v(ts)
+    rc = lib().orc_coupled_step(_p(U), nx, ny, nz, dx, dy, dz, dt, gamma, _bc(bc), rho_floor,
+                                P_floor, _p(phi), *origin, len(m), _p(p), _p(v), _p(m), _p(e),
+                                int(nsink), _p(r), _p(t), int(bool(torque_free)), rho_sink_min)
+    if rc:
+        raise ValueError(f"oracle coupled_step failed ({rc})")

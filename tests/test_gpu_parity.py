@@ -301,7 +301,7 @@ def test_every_tile_shape_gives_the_same_bits():
             if ref is None:
                 ref = out
             np.testing.assert_array_equal(out, ref)
-    for sz in range(4):
+    for sz in range(8):
         g = bsn.Grid(*s).set_state(U)
         g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
         b = bsn.Batch([g])
@@ -401,7 +401,7 @@ def test_fused_and_inplace_step_forms_agree(prec):
     shapes = [(40, 24, 36), (17, 33, 9)]
     out = {}
     # in-place form, then the fused form with every update-kernel shape
-    for form in ("inplace", 0, 1, 2, 3):
+    for form in ("inplace", 0, 1, 2, 3, 4, 5, 6, 7):
         grids = []
         for i, s in enumerate(shapes):
             U = random_state(*s, seed=50 + i)
@@ -414,7 +414,7 @@ def test_fused_and_inplace_step_forms_agree(prec):
             b.set_update_shape(form)
         b.run(0.4, 4, graph=False)
         out[form] = [g.get_state() for g in grids]
-    for form in (0, 1, 2, 3):
+    for form in (0, 1, 2, 3, 4, 5, 6, 7):
         for a, c in zip(out[form], out["inplace"]):
             np.testing.assert_array_equal(a, c)
 
@@ -446,3 +446,72 @@ def test_golden_fixtures_bitwise():
     bsn.fmr3d_step_(G, dt)
     np.testing.assert_array_equal(G.coarse.get_state(), f["coarse_out"])
     np.testing.assert_array_equal(G.fine.get_state(), f["fine_out"])
+
+
+def _bodies():
+    from binary_supernova_amd.sources import BlackHole
+    return [BlackHole([0.13, -0.02, 0.05], [0.3, -0.1, 0.2], 0.7, 0.05, 1e6, 0.22),
+            BlackHole([-0.21, 0.11, -0.3], [0.0, 0.05, 0.0], 0.3, 0.02, 1e6, 0.15)]
+
+
+def test_point_mass_and_sink_sources_bitwise():
+    # add_bh_gravity_source! / add_sink_sources! (src/gravity_bh.jl:58-78,
+    # src/sinks.jl:32-58) against the oracle
+    from binary_supernova_amd.sources import r_sink, t_sink
+    print("sources", flush=True)
+    nx, ny, nz = 12, 10, 9
+    d = (0.1, 0.1, 0.1)
+    org = (-0.6, -0.5, -0.45)
+    U = random_state(nx, ny, nz, seed=61, wild=False)
+    rng = np.random.default_rng(62)
+    bhs = _bodies()
+    dU = rng.normal(size=U.shape)
+    ref = dU.copy()
+    bsn.add_bh_gravity_source_(dU, U, nx, ny, nz, *d, bhs, *org)
+    O.bh_gravity_source(ref, U, *d, *org, [b.pos for b in bhs], [b.mass for b in bhs],
+                        [b.eps for b in bhs])
+    np.testing.assert_array_equal(dU, ref)
+    for tf in (True, False):
+        dU = rng.normal(size=U.shape)
+        ref = dU.copy()
+        before = dU.copy()
+        bsn.add_sink_sources_(dU, U, nx, ny, nz, *d, bhs, *org, f_sink=0.7, torque_free=tf,
+                              rho_sink_min=0.5)
+        O.sink_sources(ref, U, *d, *org, [b.pos for b in bhs], [b.vel for b in bhs],
+                       [r_sink(b) for b in bhs], [t_sink(b, 0.7) for b in bhs], tf, 0.5)
+        assert (ref[..., 0] != before[..., 0]).sum() > 20      # the sinks cover cells
+        np.testing.assert_array_equal(dU, ref)
+
+
+@pytest.mark.parametrize("form", ["fused", "inplace", "single"])
+def test_step_with_point_masses_sinks_and_phi_bitwise(form):
+    # the hydrodynamic part of the coupled step (scripts/run_sn50_fiducial.jl:
+    # 772-816) with a frozen phi, against the oracle
+    from binary_supernova_amd.sources import r_sink, set_point_masses, set_sinks, t_sink
+    print(f"coupled step {form}", flush=True)
+    nx, ny, nz = 14, 12, 10
+    d = (0.1, 0.1, 0.1)
+    org = (-0.7, -0.6, -0.5)
+    U = random_state(nx, ny, nz, seed=71, wild=False)
+    phi = np.random.default_rng(72).normal(size=(nz, ny, nx))
+    bhs = _bodies()
+    g = bsn.Grid(nx, ny, nz).set_state(U)
+    g.set_params(*d, 1.4, "outflow", 1e-4, 1e-4)
+    g.set_phi(phi)
+    set_point_masses(g, bhs, *org)
+    set_sinks(g, bhs[:1], *org, f_sink=0.8, torque_free=True, rho_sink_min=0.3)
+    V = U.copy()
+    for dt in (2e-3, 1.5e-3):
+        if form == "single":
+            bsn.euler3d_step_(g, nx, ny, nz, *d, dt, 1.4, bc="outflow", rho_floor=1e-4,
+                              P_floor=1e-4)
+        else:
+            b = bsn.Batch([g])
+            b.set_mode(form == "inplace")
+            b.step([dt])
+        O.coupled_step(V, *d, dt, 1.4, "outflow", 1e-4, 1e-4, phi=phi, origin=org,
+                       pos=[b_.pos for b_ in bhs], vel=[b_.vel for b_ in bhs],
+                       mass=[b_.mass for b_ in bhs], eps=[b_.eps for b_ in bhs], nsink=1,
+                       rs=[r_sink(bhs[0])], ts=[t_sink(bhs[0], 0.8)], torque_free=True,
+                       rho_sink_min=0.3)
+    np.testing.assert_array_equal(g.get_state(), V)
