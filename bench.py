@@ -43,12 +43,14 @@ VALU_FP32_TFLOPS = 157.3
 # energies 24, HLLC 130, cell pressure 10.
 FLOPS_PER_INTERFACE = 784
 # Algorithmic HBM bytes per cell of the fused z-sweep + stage-update kernel
-# (k_zupdate), per launch: read the stage input U and the RK3 base state Un
-# (5 doubles each), the x and y fluxes (5 words each, one word per interface
-# and field: the face shared by two cells counts once) and write the new
-# state (5 doubles).  The z fluxes never leave registers.
-def update_bytes_per_cell(prec: str) -> int:
-    return 3 * 5 * 8 + 2 * 5 * (8 if prec == "fp64" else 4)
+# (k_zupdate), averaged over the three launches of a step: every stage reads
+# its input state (5 doubles), the x and y fluxes (5 words each: the face
+# shared by two cells counts once) and writes the new state (5 doubles);
+# stages 2 and 3 also read the RK3 base state (5 doubles), which in stage 1
+# is the input itself.  The z fluxes never leave registers.
+def update_bytes_per_cell(prec: str) -> float:
+    w = 8 if prec == "fp64" else 4
+    return 2 * 5 * 8 + 2 * 5 * w + (2.0 / 3.0) * 5 * 8
 
 
 def profiled_traffic(kernel: str, prec: str):
