@@ -40,10 +40,10 @@ class Batch:
         call("bsn_batch_set_shapes", self.handle, sx, sy, sz)
 
     def set_update_shape(self, shape: int) -> None:
-        """Fused z-sweep + update kernel shape 0..7: even/odd = 16/32 cells per
-        thread along z; 2/3 load operands one cell ahead; 4/5 cap registers
-        for 4 waves per SIMD (split precision); 6/7 keep the window's raw
-        cells in an LDS ring."""
+        """Fused z-sweep + update kernel shape 0..11 = 3 * variant +
+        segmentation: variant 0 plain, 1 operands loaded one cell ahead, 2 LDS
+        ring of the window's raw cells, 3 ring + early window load;
+        segmentation 0/1 = z runs of 16/32 cells, 2 = balanced run lengths."""
         call("bsn_batch_set_update_shape", self.handle, int(shape))
 
     def set_mode(self, inplace: bool) -> None:

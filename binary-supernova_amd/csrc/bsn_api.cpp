@@ -166,13 +166,43 @@ static std::vector<Tile> update_tiles(const std::vector<GridDev>& G) {
     return t;
 }
 
-static std::vector<Tile> zupdate_tiles(const std::vector<GridDev>& G, int kz) {
+// Column tiles of the fused z-sweep + update kernel: 64 x 4 columns of one
+// grid and a run of cells along z.  seg 0/1: runs of 16/32 cells.  seg 2: the
+// runs of each grid have one length (split as evenly as the grid's nz allows),
+// chosen so that the tile count is close to a whole multiple of the slots
+// (resident workgroups) and no run exceeds 96 cells: the persistent launch
+// then gives every workgroup the same work, and long runs amortise the
+// five-cell window start-up.
+static std::vector<Tile> zupdate_tiles(const std::vector<GridDev>& G, int seg, int slots) {
     std::vector<Tile> t;
+    std::vector<int> nseg(G.size(), 1);
+    if (seg == 2) {
+        double work = 0;    // column-cells
+        for (const GridDev& d : G)
+            work += (double)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz;
+        const double s = slots > 0 ? slots : 1;
+        int m = 1;
+        while (work / (m * s) > 96.0) ++m;
+        const double Lt = work / (m * s);
+        for (size_t g = 0; g < G.size(); ++g) {
+            int n = (int)(G[g].nz / Lt + 0.5);
+            nseg[g] = n < 1 ? 1 : n;
+        }
+    } else {
+        const int kz = seg == 1 ? 32 : 16;
+        for (size_t g = 0; g < G.size(); ++g) nseg[g] = (G[g].nz + kz - 1) / kz;
+    }
     for (int g = 0; g < (int)G.size(); ++g) {
         const GridDev& d = G[g];
-        for (int k0 = 0; k0 < d.nz; k0 += kz)
+        int n = nseg[g];
+        while ((d.nz + n - 1) / n > ZT_MAX_LEN) ++n;
+        for (int s = 0; s < n; ++s) {
+            const int k0 = (int)((int64_t)d.nz * s / n), k1 = (int)((int64_t)d.nz * (s + 1) / n);
+            if (k1 <= k0) continue;
+            const int c = k0 | ((k1 - k0) << ZT_LEN_SHIFT);
             for (int j0 = 0; j0 < d.ny; j0 += 4)
-                for (int i0 = 0; i0 < d.nx; i0 += 64) t.push_back({g, i0, j0, k0});
+                for (int i0 = 0; i0 < d.nx; i0 += 64) t.push_back({g, i0, j0, c});
+        }
     }
     return t;
 }
@@ -250,7 +280,7 @@ static int plan_refresh(Plan& P) {
 
 static int plan_set_zshape(Plan& P, int shape) {
     auto G = host_descs(P);
-    auto t = zupdate_tiles(G, zupdate_kz(shape));
+    auto t = zupdate_tiles(G, zupdate_seg(shape), zupdate_slots(P.prec, shape, P.ctx->cus));
     CHK(upload_vec(&P.d_zt, t));
     P.nzt = (int)t.size();
     P.zshape = shape;
@@ -1044,15 +1074,16 @@ extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             ms /= reps;
-            static const char* var[4] = {"", "pf", "w4", "ring"};
-            snprintf(buf, sizeof buf, "%s\"kz%d%s\": %.4f", s ? ", " : "", zupdate_kz(s),
-                     var[s >> 1], ms);
+            static const char* var[4] = {"", "pf", "ring", "ringpf"};
+            static const char* sg[3] = {"kz16", "kz32", "kbal"};
+            snprintf(buf, sizeof buf, "%s\"%s%s\": %.4f", s ? ", " : "", sg[s % 3], var[s / 3], ms);
             rep += buf;
             if (ms < best_ms) { best_ms = ms; best = s; }
         }
         CHK(plan_set_zshape(P, best));
-        static const char* vb[4] = {"", "pf", "w4", "ring"};
-        snprintf(buf, sizeof buf, ", \"best\": \"kz%d%s\"}", zupdate_kz(best), vb[best >> 1]);
+        static const char* vb[4] = {"", "pf", "ring", "ringpf"};
+        static const char* sb[3] = {"kz16", "kz32", "kbal"};
+        snprintf(buf, sizeof buf, ", \"best\": \"%s%s\"}", sb[best % 3], vb[best / 3]);
         rep += buf;
     }
     rep += "}";

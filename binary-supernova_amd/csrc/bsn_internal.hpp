@@ -79,6 +79,11 @@ struct Tile {
     int g, a, b, c;
 };
 
+// Fused z-sweep + update tiles pack the first cell and the column length in c.
+#define ZT_K0_MASK 0x1FFFFF
+#define ZT_LEN_SHIFT 21
+#define ZT_MAX_LEN 1023
+
 // FMR coupling between a parent grid and the nested child grid.
 struct FmrDev {
     int parent, child;        // grid indices into the GridDev table

@@ -543,20 +543,21 @@ __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grid
     }
 }
 
-// z sweep fused with the stage update.  Tile (g, i0, j0, k0): 64 x 4 threads
-// over (i, j); each thread walks its z column from k0 for KZ cells with a
+// z sweep fused with the stage update.  Tile (g, i0, j0, k0 | len): 64 x 4
+// threads over (i, j); each thread walks its z column from k0 for len cells with a
 // six-cell register window, computes the z interface fluxes in order and
 // finishes every cell as soon as both of its z faces are known, reading the
 // x and y fluxes from HBM.  The z fluxes never leave registers.  The stage
 // writes Uout != U, so columns can be walked in any order.
-//   PF:   the next window cell and the update operands are loaded before the
-//         flux arithmetic (latency overlap, more registers);
+//   PF:   1 = the next window cell and the update operands are loaded before
+//         the flux arithmetic (latency overlap, more registers); 2 = only the
+//         next window cell is;
 //   WB:   minimum waves per SIMD the register allocation must allow;
 //   RING: the raw FP64 values of the last four window cells stay in a
 //         per-thread LDS ring, so the update reads the cell's own state (and,
 //         when Un is the stage input, its base state) from LDS instead of
 //         fetching it from HBM a second time.
-template <typename T, int KZ, int PF, int WB, int RING>
+template <typename T, int PF, int WB, int RING>
 __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__ grids,
                                                      const Tile* __restrict__ tiles, int ntiles,
                                                      int stage, int do_cfl) {
@@ -568,10 +569,12 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
     for (int t = t0; t < t1; ++t) {
         const Tile tl = tiles[t];
         const GridDev& G = grids[tl.g];
-        const int i = tl.a + lx, j = tl.b + ly, k0 = tl.c;
+        // tile.c packs the first cell k0 (bits 0-20) and the column length (bits 21-30)
+        const int k0 = tl.c & ZT_K0_MASK, klen = tl.c >> ZT_LEN_SHIFT;
+        const int i = tl.a + lx, j = tl.b + ly;
         uint64_t smax = 0;
         if (i < G.nx && j < G.ny) {
-            const int kend = min(k0 + KZ, G.nz);
+            const int kend = min(k0 + klen, G.nz);
             const T* Fx = (const T*)G.F[0];
             const T* Fy = (const T*)G.F[1];
             T c[4][6], p[6];
@@ -633,13 +636,13 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
             double fz0[5];
             for (int f = k0; f <= kend; ++f) {
                 place(5);                               // cell f+5
-                if (PF && f < kend) fetch(f + 6);       // next iteration's cell, in flight during the flux
+                if (PF >= 1 && f < kend) fetch(f + 6);  // next iteration's cell, in flight during the flux
                 const int k = f - 1;                    // the cell finished in this iteration
-                if (PF && f > k0) operands(k);
+                if (PF == 1 && f > k0) operands(k);
                 T F[5];
                 Prec<T>::template flux<2>(c, p, G, F);
-                if (!PF && f < kend) fetch(f + 6);
-                if (!PF && f > k0) operands(k);
+                if (PF == 0 && f < kend) fetch(f + 6);
+                if (PF != 1 && f > k0) operands(k);
                 if (f > k0) {
                     double d[5];
 #pragma unroll
@@ -1224,36 +1227,60 @@ hipError_t launch_update(const LaunchCtx& L, int prec, const GridDev* grids, con
 
 static int flat_blocks(int64_t n, int cus);
 
-template <typename T, int KZ, int PF, int WB, int RING>
+template <typename T, int PF, int WB, int RING>
 static hipError_t launch_zupdate_t(const LaunchCtx& L, const GridDev* grids, const Tile* tiles,
                                    int ntiles, int stage, int do_cfl) {
-    const void* fn = (const void*)&k_zupdate<T, KZ, PF, WB, RING>;
+    const void* fn = (const void*)&k_zupdate<T, PF, WB, RING>;
     const int g = occupancy_grid(fn, 256, 0, ntiles, L.cus);
-    hipLaunchKernelGGL((k_zupdate<T, KZ, PF, WB, RING>), dim3(g), dim3(256), 0, L.stream, grids,
+    hipLaunchKernelGGL((k_zupdate<T, PF, WB, RING>), dim3(g), dim3(256), 0, L.stream, grids,
                        tiles, ntiles, stage, do_cfl);
     return hipGetLastError();
 }
 
-// Shapes (bit 0: 16 or 32 cells per thread along z; bits 1-2: variant):
-// variant 0 plain, 1 operands loaded one cell ahead, 2 register budget
-// capped for 4 waves per SIMD (FP32 flux only; the FP64 form keeps variant
-// 0), 3 LDS ring of the window's raw cells.
-int zupdate_kz(int shape) { return (shape & 1) ? 32 : 16; }
-int zupdate_shape_count() { return 8; }
+template <typename T, int PF, int WB, int RING>
+static int zupdate_slots_t(int cus) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)&k_zupdate<T, PF, WB, RING>,
+                                                     256, 0) != hipSuccess || per < 1)
+        per = 1;
+    return per * cus;
+}
+
+// Shapes: 3 * variant + segmentation.  Variant 0 plain, 1 operands loaded one
+// cell ahead, 2 LDS ring of the window's raw cells, 3 ring plus an early load
+// of the next window cell.  Segmentation 0: 16 cells per column tile, 1: 32,
+// 2: one length per grid chosen so the tile count fills the resident
+// workgroup slots (zupdate_slots) evenly.
+int zupdate_shape_count() { return 12; }
+int zupdate_seg(int shape) { return shape % 3; }
+
+int zupdate_slots(int prec, int shape, int cus) {
+    const int v = shape / 3;
+    if (prec == 0) {
+        switch (v) {
+            case 1: return zupdate_slots_t<double, 1, 1, 0>(cus);
+            case 2: return zupdate_slots_t<double, 0, 1, 1>(cus);
+            case 3: return zupdate_slots_t<double, 2, 1, 1>(cus);
+            default: return zupdate_slots_t<double, 0, 1, 0>(cus);
+        }
+    }
+    switch (v) {
+        case 1: return zupdate_slots_t<float, 1, 3, 0>(cus);
+        case 2: return zupdate_slots_t<float, 0, 1, 1>(cus);
+        case 3: return zupdate_slots_t<float, 2, 1, 1>(cus);
+        default: return zupdate_slots_t<float, 0, 1, 0>(cus);
+    }
+}
 
 template <typename T>
 static hipError_t launch_zupdate_p(const LaunchCtx& L, int shape, const GridDev* grids,
                                    const Tile* tiles, int ntiles, int stage, int do_cfl) {
     constexpr bool F = sizeof(T) == 4;
-    switch (shape) {
-        case 1: return launch_zupdate_t<T, 32, 0, 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
-        case 2: return launch_zupdate_t<T, 16, 1, F ? 3 : 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
-        case 3: return launch_zupdate_t<T, 32, 1, F ? 3 : 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
-        case 4: return launch_zupdate_t<T, 16, 0, F ? 4 : 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
-        case 5: return launch_zupdate_t<T, 32, 0, F ? 4 : 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
-        case 6: return launch_zupdate_t<T, 16, 0, 1, 1>(L, grids, tiles, ntiles, stage, do_cfl);
-        case 7: return launch_zupdate_t<T, 32, 0, 1, 1>(L, grids, tiles, ntiles, stage, do_cfl);
-        default: return launch_zupdate_t<T, 16, 0, 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+    switch (shape / 3) {
+        case 1: return launch_zupdate_t<T, 1, F ? 3 : 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 2: return launch_zupdate_t<T, 0, 1, 1>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 3: return launch_zupdate_t<T, 2, 1, 1>(L, grids, tiles, ntiles, stage, do_cfl);
+        default: return launch_zupdate_t<T, 0, 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
     }
 }
 

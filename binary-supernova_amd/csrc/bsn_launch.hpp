@@ -20,8 +20,9 @@ hipError_t launch_flux(const LaunchCtx& L, int dir, int prec, int shape, const G
 hipError_t launch_update(const LaunchCtx& L, int prec, const GridDev* grids, const Tile* tiles,
                          int ntiles, int stage, int do_cfl, double* dU_out);
 hipError_t launch_cfl(const LaunchCtx& L, const GridDev* grids, const Tile* tiles, int ntiles);
-int zupdate_kz(int shape);
 int zupdate_shape_count();
+int zupdate_seg(int shape);
+int zupdate_slots(int prec, int shape, int cus);
 hipError_t launch_zupdate(const LaunchCtx& L, int prec, int shape, const GridDev* grids,
                           const Tile* tiles, int ntiles, int stage, int do_cfl);
 hipError_t launch_ghost_fill(const LaunchCtx& L, const GridDev* grids, const int64_t* off,

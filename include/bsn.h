@@ -153,11 +153,13 @@ int bsn_batch_destroy(bsn_batch* b);
 int bsn_batch_autotune(bsn_batch* b, char* report, int report_len);
 /* Sets the flux tile shape ids explicitly (x: 0..2, y/z: 0..3). */
 int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz);
-/* Sets the shape id (0..7) of the fused z-sweep + stage-update kernel used by
- * the out-of-place step: even = 16 cells per thread along z, odd = 32; 2 and 3
- * load the next cell and the update operands one cell ahead; 4 and 5 cap the
- * register budget at 4 waves per SIMD (split precision); 6 and 7 keep the
- * window's raw cells in an LDS ring instead of reading them twice. */
+/* Sets the shape id (0..11) of the fused z-sweep + stage-update kernel used
+ * by the out-of-place step: shape = 3 * variant + segmentation.  Variant 0
+ * plain, 1 loads the next cell and the update operands one cell ahead, 2 keeps
+ * the window's raw cells in an LDS ring instead of reading them twice, 3 the
+ * ring plus an early load of the next window cell.  Segmentation 0 and 1: z
+ * runs of 16 or 32 cells per column tile; 2: one run length per grid chosen
+ * so that the tiles fill the resident workgroups evenly. */
 int bsn_batch_set_update_shape(bsn_batch* b, int shape);
 /* Step form: 0 (default) = out of place, z sweep fused with the stage update;
  * 1 = in place with all fluxes materialised.  Both give the same bits. */

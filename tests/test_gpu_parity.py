@@ -301,7 +301,7 @@ def test_every_tile_shape_gives_the_same_bits():
             if ref is None:
                 ref = out
             np.testing.assert_array_equal(out, ref)
-    for sz in range(8):
+    for sz in range(12):
         g = bsn.Grid(*s).set_state(U)
         g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
         b = bsn.Batch([g])
@@ -401,7 +401,7 @@ def test_fused_and_inplace_step_forms_agree(prec):
     shapes = [(40, 24, 36), (17, 33, 9)]
     out = {}
     # in-place form, then the fused form with every update-kernel shape
-    for form in ("inplace", 0, 1, 2, 3, 4, 5, 6, 7):
+    for form in ("inplace", *range(12)):
         grids = []
         for i, s in enumerate(shapes):
             U = random_state(*s, seed=50 + i)
@@ -414,7 +414,7 @@ def test_fused_and_inplace_step_forms_agree(prec):
             b.set_update_shape(form)
         b.run(0.4, 4, graph=False)
         out[form] = [g.get_state() for g in grids]
-    for form in (0, 1, 2, 3, 4, 5, 6, 7):
+    for form in range(12):
         for a, c in zip(out[form], out["inplace"]):
             np.testing.assert_array_equal(a, c)
 
