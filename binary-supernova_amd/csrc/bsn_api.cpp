@@ -512,6 +512,9 @@ extern "C" int bsn_grid_create(bsn_ctx* ctx, int64_t nx, int64_t ny, int64_t nz,
     h.ffs[1] = round_up((int64_t)h.fp[1] * (h.ny + 1) * h.nz, 128);
     h.ffs[2] = round_up((int64_t)h.fp[2] * h.ny * (h.nz + 1), 128);
     h.gamma = 1.4; h.gfac = (h.gamma + 1.0) / (2.0 * h.gamma);
+    h.gf32[0] = (float)h.gamma;
+    h.gf32[1] = (float)h.gfac;
+    h.gf32[2] = (float)(1.0 / (h.gamma - 1.0));
     h.idx = h.idy = h.idz = 1.0;
     h.bc = 0;
     auto fail = [&](hipError_t e) {
@@ -787,6 +790,9 @@ static int apply_params(bsn_grid* g, double dx, double dy, double dz, double gam
     g->dx = dx; g->dy = dy; g->dz = dz;
     h.idx = idx; h.idy = idy; h.idz = idz;
     h.gamma = gamma; h.gfac = gfac;
+    h.gf32[0] = (float)gamma;
+    h.gf32[1] = (float)gfac;
+    h.gf32[2] = (float)(1.0 / (gamma - 1.0));
     h.bc = bc;
     h.rho_floor = rho_floor;
     h.P_floor = P_floor;

@@ -71,6 +71,7 @@ struct GridDev {
                        // needed); bit 1 = active cells are floored on load
     double idx, idy, idz;
     double gamma, gfac, rho_floor, P_floor;
+    float gf32[3];     // gamma, gfac and 1/(gamma - 1) rounded to FP32 (split flux)
 };
 
 // A tile of work: which grid, and three integer coordinates whose meaning is

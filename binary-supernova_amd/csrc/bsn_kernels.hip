@@ -227,7 +227,7 @@ template <> struct Prec<float> {
     template <int DIR>
     __device__ static void flux(const float (&c)[4][6], const float (&p)[6], const GridDev& G,
                                 float F[5]) {
-        sp::interface_flux<DIR>(c, p, (float)G.gamma, (float)G.gfac, (float)(1.0 / (G.gamma - 1.0)), F);
+        sp::interface_flux<DIR>(c, p, G.gf32[0], G.gf32[1], G.gf32[2], F);
     }
 };
 
@@ -241,18 +241,28 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
     const int tid = threadIdx.x;
     int t0, t1;
     chunk(ntiles, t0, t1);
+    // interface f along DIR with cross indices ci, cj (active, 0-based)
+    auto emit = [&](const GridDev& G, const T (&c)[4][6], const T (&p)[6], int f, int ci, int cj) {
+        T F[5];
+        Prec<T>::template flux<DIR>(c, p, G, F);
+        T* Fp = (T*)G.F[DIR];
+        int64_t o;
+        if (DIR == 0)      o = ((int64_t)cj * G.ny + ci) * G.fp[0] + f;
+        else if (DIR == 1) o = ((int64_t)cj * (G.ny + 1) + f) * G.fp[1] + ci;
+        else               o = ((int64_t)f * G.ny + cj) * G.fp[2] + ci;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) Fp[q * G.ffs[DIR] + o] = F[q];
+    };
     for (int t = t0; t < t1; ++t) {
         const Tile tl = tiles[t];
         const GridDev& G = grids[tl.g];
         const int Nx = G.nx + 2 * NG;
         T c[4][6], p[6];
-        bool valid;
-        int f, ci, cj;   // interface index along DIR, and the two cross indices (active, 0-based)
         if (DIR == 0 && tl.a < 0) {
-            // last x interface f = nx of rows j0 + tid, gathered from HBM
-            f = G.nx; ci = tl.b + tid; cj = tl.c;
-            valid = ci < G.ny;
-            if (valid) {
+            // last x interface f = nx of rows j0 + tid, gathered from HBM; the
+            // tile is uniform, so every thread of the block takes this branch
+            const int ci = tl.b + tid, cj = tl.c;
+            if (ci < G.ny) {
 #pragma unroll
                 for (int m = 0; m < 6; ++m) {
                     double u[5];
@@ -262,61 +272,54 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
                     for (int q = 0; q < 4; ++q) c[q][m] = (T)u[q];
                     p[m] = (T)pr;
                 }
+                emit(G, c, p, G.nx, ci, cj);
             }
-        } else {
-            // stage cells into LDS
-            for (int e = tid; e < SA * SB; e += NT) {
-                const int a = e / SB, b = e - a * SB;
-                int pi, pj, pk;
-                bool ok;
-                if (DIR == 0) {
-                    pi = tl.a + b; pj = NG + tl.b + a; pk = NG + tl.c;
-                    ok = pi < Nx && tl.b + a < G.ny;
-                } else if (DIR == 1) {
-                    pi = NG + tl.b + b; pj = tl.a + a; pk = NG + tl.c;
-                    ok = tl.b + b < G.nx && pj < G.ny + 2 * NG;
-                } else {
-                    pi = NG + tl.b + b; pj = NG + tl.c; pk = tl.a + a;
-                    ok = tl.b + b < G.nx && pk < G.nz + 2 * NG;
-                }
-                if (ok) {
-                    double u[5];
-                    load_cell<DIR>(G, pi, pj, pk, u);
-                    const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) lds[q][a][b] = (T)u[q];
-                    lds[4][a][b] = (T)pr;
-                }
-            }
-            __syncthreads();
-            const int x = tid % W, h = tid / W;
+            continue;
+        }
+        // stage cells into LDS
+        for (int e = tid; e < SA * SB; e += NT) {
+            const int a = e / SB, b = e - a * SB;
+            int pi, pj, pk;
+            bool ok;
             if (DIR == 0) {
-                f = tl.a + x; ci = tl.b + h; cj = tl.c;
-                valid = f < G.nx && ci < G.ny;
+                pi = tl.a + b; pj = NG + tl.b + a; pk = NG + tl.c;
+                ok = pi < Nx && tl.b + a < G.ny;
+            } else if (DIR == 1) {
+                pi = NG + tl.b + b; pj = tl.a + a; pk = NG + tl.c;
+                ok = tl.b + b < G.nx && pj < G.ny + 2 * NG;
             } else {
-                f = tl.a + h; ci = tl.b + x; cj = tl.c;
-                valid = f <= (DIR == 1 ? G.ny : G.nz) && ci < G.nx;
+                pi = NG + tl.b + b; pj = NG + tl.c; pk = tl.a + a;
+                ok = tl.b + b < G.nx && pk < G.nz + 2 * NG;
             }
-            if (valid) {
+            if (ok) {
+                double u[5];
+                load_cell<DIR>(G, pi, pj, pk, u);
+                const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
 #pragma unroll
-                for (int m = 0; m < 6; ++m) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        c[q][m] = DIR == 0 ? lds[q][h][x + m] : lds[q][h + m][x];
-                    p[m] = DIR == 0 ? lds[4][h][x + m] : lds[4][h + m][x];
-                }
+                for (int q = 0; q < 4; ++q) lds[q][a][b] = (T)u[q];
+                lds[4][a][b] = (T)pr;
             }
         }
+        __syncthreads();
+        const int x = tid % W, h = tid / W;
+        int f, ci;
+        bool valid;
+        if (DIR == 0) {
+            f = tl.a + x; ci = tl.b + h;
+            valid = f < G.nx && ci < G.ny;
+        } else {
+            f = tl.a + h; ci = tl.b + x;
+            valid = f <= (DIR == 1 ? G.ny : G.nz) && ci < G.nx;
+        }
         if (valid) {
-            T F[5];
-            Prec<T>::template flux<DIR>(c, p, G, F);
-            T* Fp = (T*)G.F[DIR];
-            int64_t o;
-            if (DIR == 0)      o = ((int64_t)cj * G.ny + ci) * G.fp[0] + f;
-            else if (DIR == 1) o = ((int64_t)cj * (G.ny + 1) + f) * G.fp[1] + ci;
-            else               o = ((int64_t)f * G.ny + cj) * G.fp[2] + ci;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) Fp[q * G.ffs[DIR] + o] = F[q];
+            for (int m = 0; m < 6; ++m) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    c[q][m] = DIR == 0 ? lds[q][h][x + m] : lds[q][h + m][x];
+                p[m] = DIR == 0 ? lds[4][h][x + m] : lds[4][h + m][x];
+            }
+            emit(G, c, p, f, ci, tl.c);
         }
         __syncthreads();
     }
