@@ -258,21 +258,25 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
         const GridDev& G = grids[tl.g];
         const int Nx = G.nx + 2 * NG;
         T c[4][6], p[6];
-        if (DIR == 0 && tl.a < 0) {
-            // last x interface f = nx of rows j0 + tid, gathered from HBM; the
-            // tile is uniform, so every thread of the block takes this branch
+        if (tl.a < 0) {
+            // edge tile (x only): the last interface f = nx of NT consecutive
+            // rows from b, gathered from HBM.  The tile is uniform, so every
+            // thread of the block takes this branch.
             const int ci = tl.b + tid, cj = tl.c;
-            if (ci < G.ny) {
+            const int f = DIR == 0 ? G.nx : DIR == 1 ? G.ny : G.nz;
+            if (ci < (DIR == 0 ? G.ny : G.nx)) {
 #pragma unroll
                 for (int m = 0; m < 6; ++m) {
                     double u[5];
-                    load_cell<0>(G, G.nx + m, NG + ci, NG + cj, u);
+                    if (DIR == 0)      load_cell<0>(G, f + m, NG + ci, NG + cj, u);
+                    else if (DIR == 1) load_cell<1>(G, NG + ci, f + m, NG + cj, u);
+                    else               load_cell<2>(G, NG + ci, NG + cj, f + m, u);
                     const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) c[q][m] = (T)u[q];
                     p[m] = (T)pr;
                 }
-                emit(G, c, p, G.nx, ci, cj);
+                emit(G, c, p, f, ci, cj);
             }
             continue;
         }
