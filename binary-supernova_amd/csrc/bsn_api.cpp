@@ -1287,6 +1287,16 @@ extern "C" int bsn_fmr_create(bsn_ctx* ctx, int nlevels, bsn_grid* const* levels
             return fail(set_err(BSN_ERR_HIP, "flux register allocation"));
         f.Ucn = levels[l]->h.Un;
         f.Ucnp1 = levels[l]->h.U;
+        // the parent cells any prolongation stencil of the child reads (the
+        // same first / last fine index bound as the box check above)
+        for (int a = 0; a < 3; ++a) {
+            const GridDev& ch = levels[l + 1]->h;
+            const int nca = a == 0 ? ch.nx : a == 1 ? ch.ny : ch.nz;
+            f.plo[a] = BSN_NG + coarse_centre(1, f.lo[a], f.ratio) - 3;
+            f.phi[a] = BSN_NG + coarse_centre(nca + 2 * BSN_NG, f.lo[a], f.ratio) + 1;
+        }
+        if (hipMalloc((void**)&f.Uci, 5 * levels[l]->h.fs * 8) != hipSuccess)
+            return fail(set_err(BSN_ERR_HIP, "interpolation buffer allocation"));
         m->hf.push_back(f);
     }
     if (!m->hf.empty() && upload_vec(&m->d_f, m->hf) != BSN_OK) return fail(BSN_ERR_HIP);
@@ -1300,7 +1310,7 @@ extern "C" int bsn_fmr_destroy(bsn_fmr* m) {
     (void)hipStreamSynchronize(m->ctx->stream);
     graph_drop(m->gexec, m->graph);
     for (auto* P : m->plans) plan_free(P);
-    for (auto& f : m->hf) { (void)hipFree(f.frc); (void)hipFree(f.frf); }
+    for (auto& f : m->hf) { (void)hipFree(f.frc); (void)hipFree(f.frf); (void)hipFree(f.Uci); }
     (void)hipFree(m->d_f);
     (void)hipFree(m->d_all);
     delete m;
@@ -1319,12 +1329,24 @@ static int64_t ghost_total(const GridDev& d) {
     return 2 * BSN_NG * ((int64_t)d.Ny * d.Nz + (int64_t)d.nx * d.Nz + (int64_t)d.nx * d.ny);
 }
 
+// interp: from the parent state interpolated in time (fmr_interp), else from
+// the parent's current state.
 static int fmr_prolong(bsn_fmr* m, int l, int all_cells, bool interp) {
     const GridDev& C = m->lev[l]->h;
     const GridDev& Pg = m->lev[l - 1]->h;
     const int64_t total = all_cells ? (int64_t)(C.nx + 2 * BSN_NG) * C.Ny * C.Nz : ghost_total(C);
     HIPCHK(launch_prolong(m->ctx->L(), m->d_all, m->d_f + (l - 1), total,
-                          interp ? Pg.Un : Pg.U, interp ? Pg.U : nullptr, m->alpha[l], all_cells));
+                          interp ? m->hf[l - 1].Uci : Pg.U, all_cells));
+    return BSN_OK;
+}
+
+// The parent state of level l at time fraction alpha[l] of its step, over
+// the cells the prolongation reads; once per sub-cycle of level l.
+static int fmr_interp(bsn_fmr* m, int l) {
+    const FmrDev& f = m->hf[l - 1];
+    const int64_t total = (int64_t)(f.phi[0] - f.plo[0] + 1) * (f.phi[1] - f.plo[1] + 1) *
+                          (f.phi[2] - f.plo[2] + 1);
+    HIPCHK(launch_time_interp(m->ctx->L(), m->d_all, m->d_f + (l - 1), total, m->alpha[l]));
     return BSN_OK;
 }
 
@@ -1339,6 +1361,7 @@ static int fmr_advance(bsn_fmr* m, int l) {
     const GridDev& h = m->lev[l]->h;
     if (child) HIPCHK(launch_fr_zero(L, m->d_f + l, m->hf[l].froff[6]));
     if (l > 0) HIPCHK(launch_copy_state(L, h, h.U, h.Un));
+    if (l > 0) CHK(fmr_interp(m, l));
     for (int s = 1; s <= 3; ++s) {
         if (l == 0) {
             if (s == 1) { CHK(plan_ghost_fill(P, 1)); CHK(plan_floors(P, 1)); }

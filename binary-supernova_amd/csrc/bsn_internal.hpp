@@ -96,4 +96,6 @@ struct FmrDev {
     int64_t froff[7];         // face offsets into a register (in doubles)
     double* Ucn;              // parent state at the start of its step
     double* Ucnp1;            // parent state at the end of its RK3 step
+    double* Uci;              // parent state interpolated in time (parent layout)
+    int plo[3], phi[3];       // padded parent cells any prolongation stencil reads
 };

@@ -938,14 +938,12 @@ __device__ __forceinline__ void coarse_stencil(int ig_f1, int c_lo, int ratio, i
 // coarse value on the fly.  all_cells = 1 fills every fine cell
 // (_prolong_all!), else only ghost cells in the reference's slab partition.
 __global__ void k_prolong(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr,
-                          const double* __restrict__ Uca, const double* __restrict__ Ucb,
-                          double alpha, int all_cells) {
+                          const double* __restrict__ Uca, int all_cells) {
     const FmrDev& M = *fmr;
     const GridDev& C = grids[M.parent];
     const GridDev& Fg = grids[M.child];
     const int Nfx = Fg.nx + 2 * NG;
     const int64_t total = all_cells ? (int64_t)Nfx * Fg.Ny * Fg.Nz : ghost_count(Fg);
-    const double oma = 1.0 - alpha;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
@@ -961,26 +959,50 @@ __global__ void k_prolong(const GridDev* __restrict__ grids, const FmrDev* __res
         coarse_stencil(j + 1, M.lo[1], M.ratio, jc0, sj);
         coarse_stencil(k + 1, M.lo[2], M.ratio, kc0, sk);
         const int64_t d = uidx(Fg, i, j, k);
-        for (int q = 0; q < 5; ++q) {
-            const double* A = Uca + q * C.fs;
-            const double* B = Ucb ? Ucb + q * C.fs : nullptr;
-            double val = 0.0;
-            for (int di = -2; di <= 2; ++di) {
-                const int ig = NG + ic0 + di - 1;
-                const double wi = M.w[si * 5 + di + 2];
-                for (int dj = -2; dj <= 2; ++dj) {
-                    const int jg = NG + jc0 + dj - 1;
-                    const double wij = wi * M.w[sj * 5 + dj + 2];
-                    for (int dk = -2; dk <= 2; ++dk) {
-                        const int kg = NG + kc0 + dk - 1;
-                        const int64_t o = uidx(C, ig, jg, kg);
-                        const double uc = B ? oma * A[o] + alpha * B[o] : A[o];
-                        val += (wij * M.w[sk * 5 + dk + 2]) * uc;
-                    }
+        const int64_t zs = (int64_t)C.Ny * C.pitch;
+        // the five fields accumulate side by side; each field's terms are
+        // summed in the reference's (di, dj, dk) order
+        double val[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int di = -2; di <= 2; ++di) {
+            const int ig = NG + ic0 + di - 1;
+            const double wi = M.w[si * 5 + di + 2];
+            for (int dj = -2; dj <= 2; ++dj) {
+                const int jg = NG + jc0 + dj - 1;
+                const double wij = wi * M.w[sj * 5 + dj + 2];
+                const int64_t ob = uidx(C, ig, jg, NG + kc0 - 3);
+#pragma unroll
+                for (int dk = 0; dk < 5; ++dk) {
+                    const int64_t o = ob + dk * zs;
+                    const double w = wij * M.w[sk * 5 + dk];
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) val[q] += w * Uca[q * C.fs + o];
                 }
             }
-            Fg.U[q * Fg.fs + d] = val;
         }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) Fg.U[q * Fg.fs + d] = val[q];
+    }
+}
+
+// U_c_interp = (1 - alpha) * U_c_n + alpha * U_c_np1 (fmr3d.jl:547-551) over
+// the parent cells the child's prolongation stencils read, once per fine
+// sub-cycle (alpha is fixed for its three stages).
+__global__ void k_time_interp(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr,
+                              double alpha) {
+    const FmrDev& M = *fmr;
+    const GridDev& C = grids[M.parent];
+    const int nx = M.phi[0] - M.plo[0] + 1, ny = M.phi[1] - M.plo[1] + 1;
+    const int64_t total = (int64_t)nx * ny * (M.phi[2] - M.plo[2] + 1);
+    const double oma = 1.0 - alpha;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = M.plo[0] + (int)(e % nx);
+        const int64_t r = e / nx;
+        const int j = M.plo[1] + (int)(r % ny), k = M.plo[2] + (int)(r / ny);
+        const int64_t o = uidx(C, i, j, k);
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+            M.Uci[q * C.fs + o] = oma * M.Ucn[q * C.fs + o] + alpha * M.Ucnp1[q * C.fs + o];
     }
 }
 
@@ -1410,10 +1432,18 @@ hipError_t launch_totals(const LaunchCtx& L, const GridDev& G, double* part, int
 }
 
 hipError_t launch_prolong(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total,
-                          const double* Uca, const double* Ucb, double alpha, int all_cells) {
+                          const double* Uca, int all_cells) {
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_prolong, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids, fmr,
-                       Uca, Ucb, alpha, all_cells);
+                       Uca, all_cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_time_interp(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr,
+                              int64_t total, double alpha) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_time_interp, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids,
+                       fmr, alpha);
     return hipGetLastError();
 }
 

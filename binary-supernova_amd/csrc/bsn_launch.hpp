@@ -46,7 +46,9 @@ hipError_t launch_copy_state(const LaunchCtx& L, const GridDev& G, const double*
 hipError_t launch_totals(const LaunchCtx& L, const GridDev& G, double* part, int nblocks, double dV,
                          double* out);
 hipError_t launch_prolong(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total,
-                          const double* Uca, const double* Ucb, double alpha, int all_cells);
+                          const double* Uca, int all_cells);
+hipError_t launch_time_interp(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr,
+                              int64_t total, double alpha);
 hipError_t launch_restrict(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total);
 hipError_t launch_fr_zero(const LaunchCtx& L, const FmrDev* fmr, int64_t total);
 hipError_t launch_fr_coarse(const LaunchCtx& L, int prec, const GridDev* grids, const FmrDev* fmr,
