@@ -447,9 +447,8 @@ __device__ __forceinline__ void cell_state(const GridDev& G, int64_t c, int stag
 
 __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, int j, int k, int stage,
                                                      double (&d)[5], const double (&u)[5],
-                                                     const double (&un)[5], int do_cfl) {
+                                                     const double (&un)[5], int do_cfl, double dt) {
     const int64_t c = uidx(G, NG + i, NG + j, NG + k);
-    const double dt = G.scal[SCAL_DT];
     // sources in the order of the reference's coupled step
     // (scripts/run_sn50_fiducial.jl:772-781): point masses, self-gravity, sinks
     if (G.src && G.src->nbh > 0) bh_gravity_add(*G.src, i, j, k, u, d);
@@ -493,7 +492,7 @@ __device__ __forceinline__ uint64_t finish_cell(const GridDev& G, int i, int j, 
                                                 double (&d)[5], int do_cfl) {
     double u[5], un[5];
     cell_state(G, uidx(G, NG + i, NG + j, NG + k), stage, u, un);
-    return finish_cell_with(G, i, j, k, stage, d, u, un, do_cfl);
+    return finish_cell_with(G, i, j, k, stage, d, u, un, do_cfl, G.scal[SCAL_DT]);
 }
 
 template <typename T, int KZ>
@@ -582,6 +581,7 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
         uint64_t smax = 0;
         if (i < G.nx && j < G.ny) {
             const int kend = min(k0 + klen, G.nz);
+            const double dt = G.scal[SCAL_DT];
             const T* Fx = (const T*)G.F[0];
             const T* Fy = (const T*)G.F[1];
             T c[4][6], p[6];
@@ -657,7 +657,7 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
                         d[q] = (-((double)fx[1][q] - (double)fx[0][q])) * G.idx -
                                ((double)fy[1][q] - (double)fy[0][q]) * G.idy -
                                ((double)F[q] - fz0[q]) * G.idz;
-                    const uint64_t b = finish_cell_with(G, i, j, k, stage, d, u, un, do_cfl);
+                    const uint64_t b = finish_cell_with(G, i, j, k, stage, d, u, un, do_cfl, dt);
                     smax = b > smax ? b : smax;
                 }
 #pragma unroll

@@ -317,8 +317,8 @@ __device__ __forceinline__ void hllc_normal(float rL, float mnL, float mt1L, flo
     }
     EL = fmaxj(EL, KE_L);
     ER = fmaxj(ER, KE_R);
-    float vnL = mnL * irL, vt1L = mt1L * irL, vt2L = mt2L * irL;
-    float vnR = mnR * irR, vt1R = mt1R * irR, vt2R = mt2R * irR;
+    float vnL = mnL * irL;
+    float vnR = mnR * irR;
     float pL = (g - 1.0f) * (EL - KE_L);
     float pR = (g - 1.0f) * (ER - KE_R);
     float csL = fsqrt(g * pL * irL);
@@ -354,22 +354,28 @@ __device__ __forceinline__ void hllc_normal(float rL, float mnL, float mt1L, flo
         return;
     }
     float S_star = ((pR - pL) + mnL * (SL - vnL) - mnR * (SR - vnR)) * rcp(denom);
+    // Star-state jumps written through del = (S* - vn) / (S - S*), so that
+    // chi = rho (1 + del) and every jump U* - U vanishes exactly when S* = vn
+    // (identical states at rest give zero mass and energy flux in FP32 too);
+    // algebraically the reference's expressions (src/hllc.jl:140-165).
     if (S_star >= 0.0f) {
-        float chi = dL * rcp(SL - S_star);
-        float EsL = fabsf(dL) > 1e-37f ? chi * (EL * irL + (S_star - vnL) * (S_star + pL * rcp(dL))) : 0.0f;
-        F[0] = mnL + SL * (chi - rL);
+        float del = (S_star - vnL) * rcp(SL - S_star);
+        float chi = rL + rL * del;
+        float dE = fabsf(dL) > 1e-37f ? EL * del + chi * (S_star - vnL) * (S_star + pL * rcp(dL)) : -EL;
+        F[0] = mnL + SL * (rL * del);
         F[1] = FmnL + SL * (chi * S_star - mnL);
-        F[2] = mt1L * vnL + SL * (chi * vt1L - mt1L);
-        F[3] = mt2L * vnL + SL * (chi * vt2L - mt2L);
-        F[4] = FEL + SL * (EsL - EL);
+        F[2] = mt1L * vnL + SL * (mt1L * del);
+        F[3] = mt2L * vnL + SL * (mt2L * del);
+        F[4] = FEL + SL * dE;
     } else {
-        float chi = dR * rcp(SR - S_star);
-        float EsR = fabsf(dR) > 1e-37f ? chi * (ER * irR + (S_star - vnR) * (S_star + pR * rcp(dR))) : 0.0f;
-        F[0] = mnR + SR * (chi - rR);
+        float del = (S_star - vnR) * rcp(SR - S_star);
+        float chi = rR + rR * del;
+        float dE = fabsf(dR) > 1e-37f ? ER * del + chi * (S_star - vnR) * (S_star + pR * rcp(dR)) : -ER;
+        F[0] = mnR + SR * (rR * del);
         F[1] = FmnR + SR * (chi * S_star - mnR);
-        F[2] = mt1R * vnR + SR * (chi * vt1R - mt1R);
-        F[3] = mt2R * vnR + SR * (chi * vt2R - mt2R);
-        F[4] = FER + SR * (EsR - ER);
+        F[2] = mt1R * vnR + SR * (mt1R * del);
+        F[3] = mt2R * vnR + SR * (mt2R * del);
+        F[4] = FER + SR * dE;
     }
 }
 
