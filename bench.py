@@ -12,7 +12,8 @@ sigma(s) = 0.5, rms Mach 2, p = 1, CFL 0.4, K RK3 steps (default 50) with dt
 computed on the device each step.  The timed region is the split form; the
 FP64 form is run and timed the same way for the speed-up and the L1 error.
 
-Other workloads (--config): sod128, sedov128, batch48, fmr3.
+Other workloads (--config): sod128, sedov128, batch48, fmr3, selfgrav128 (the
+coupled step with an FFT Poisson solve every RK3 stage).
 
 Prints ONE JSON line.  Inputs are resident in HBM before the timed region.
 """
@@ -446,13 +447,89 @@ def bench_fmr(bsn, args):
                 energy_drift_split_base=float(abs(tB[4] - t0[4]) / t0[4]))
 
 
+def bench_selfgrav(bsn, args):
+    """A gamma = 5/3 polytrope star (R = 1, rho_c = 1) on n^3 over [-2, 2]^3
+    with gas self-gravity: every RK3 stage solves phi from its density with
+    the zero-padded hipFFT convolution (a (2n)^3 real transform pair) and adds
+    the source in the update kernel, as the coupled step does with
+    add_self_gravity_source! (scripts/run_sn50_fiducial.jl:772-816)."""
+    n = args.n or 128
+    gamma, L = 5.0 / 3.0, 4.0
+    d = (L / n,) * 3
+    floors = (1e-10, 1e-12)
+    U = bsn.problems.polytrope_ic_3d(n, n, n, *d, gamma, x0=-L / 2, y0=-L / 2, z0=-L / 2,
+                                     rho_floor=floors[0], P_floor=floors[1])
+    steps = args.steps
+    res = {}
+    for prec in ("split", "fp64"):
+        g = bsn.Grid(n, n, n, prec).set_state(U)
+        g.set_params(*d, gamma, "outflow", *floors)
+        b = bsn.Batch([g])
+        b.autotune()
+        bsn.set_self_gravity(g, True)
+        b.run(0.4, args.warmup, graph=False)
+        g.set_state(U)
+        g.ctx.synchronize()
+        t0 = time.perf_counter()
+        ms_flux, ms_upd, ms_step = b.profile(steps)
+        g.ctx.synchronize()
+        wall = time.perf_counter() - t0
+        # the same grid and steps without the solve: the difference is the
+        # cost of three Poisson solves per step
+        bsn.set_self_gravity(g, False)
+        g.set_state(U)
+        b.profile(2)
+        g.set_state(U)
+        g.ctx.synchronize()
+        _, _, ms_plain = b.profile(steps)
+        res[prec] = dict(wall=wall, ms_step=ms_step, ms_plain=ms_plain, ms_update=ms_upd)
+        log(f"self-gravity {prec}: {ms_step:.3f} ms/step, {ms_plain:.3f} ms/step without")
+    # the host-array Poisson drop-in (upload rho, solve, download phi)
+    rho = np.ascontiguousarray(U[3:-3, 3:-3, 3:-3, 0])
+    bsn.solve_poisson_isolated(rho, n, n, n, *d)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        bsn.solve_poisson_isolated(rho, n, n, n, *d)
+    ms_dropin = 1e3 * (time.perf_counter() - t0) / 5
+    r32, r64 = res["split"], res["fp64"]
+    cells = n ** 3
+    out = dict(
+        metric="zone-updates/s per RK3 step with per-stage self-gravity (split FP32/FP64)",
+        value=cells * steps / r32["wall"], unit="zone-updates/s",
+        ms_per_step=1e3 * r32["wall"] / steps, dtype="f32-flux/f64-state",
+        config=dict(workload=f"gamma 5/3 polytrope star on {n}^3 over [-2,2]^3, outflow, "
+                             f"phi from an isolated FFT Poisson solve every RK3 stage, "
+                             f"CFL 0.4, device dt", grid=[n, n, n], precision="split"),
+        fp64=dict(value=cells * steps / r64["wall"], ms_per_step=1e3 * r64["wall"] / steps),
+        poisson=dict(ms_per_solve=(r32["ms_step"] - r32["ms_plain"]) / 3,
+                     fft=f"{2 * n}^3 real-to-complex + complex-to-real, FP64 (hipFFT)",
+                     step_share=(r32["ms_step"] - r32["ms_plain"]) / r32["ms_step"],
+                     ms_step_without=r32["ms_plain"],
+                     dropin_ms_per_solve_host_arrays=ms_dropin),
+    )
+    if not args.no_cpu:
+        log("cpu baseline (one coupled step, numpy FFT Poisson)")
+        sys.path.insert(0, str(ROOT))
+        from oracle import poisson as PO
+        V = U.copy()
+        t0 = time.perf_counter()
+        PO.self_gravity_step(V, *d, 1e-4, gamma, "outflow", *floors)
+        sec = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = dict(
+            value=cells / sec, unit="zone-updates/s", cores=cores, kind="port",
+            sample=f"1 RK3 step with three numpy FFT Poisson solves of the same {n}^3 state, "
+                   f"FP64 C/OpenMP restatement, {sec:.1f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="turb256",
-                    choices=["turb256", "sedov128", "sod128", "batch48", "fmr3"])
+                    choices=["turb256", "sedov128", "sod128", "batch48", "fmr3", "selfgrav128"])
     ap.add_argument("--n", type=int, default=None, help="override the grid size")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lean", action="store_true", help="skip the hipGraph replay pass "
@@ -479,6 +556,9 @@ def main():
     elif args.config == "fmr3":
         out = bench_fmr(bsn, args)
         args.steps = args.steps or 100
+    elif args.config == "selfgrav128":
+        args.steps = args.steps or 20
+        out = bench_selfgrav(bsn, args)
     else:
         args.steps = args.steps or 100
         out = bench_sod(bsn, args)

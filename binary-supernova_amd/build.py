@@ -16,7 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsn_hip.so")
-SOURCES = ["bsn_kernels.hip", "bsn_api.cpp"]
+SOURCES = ["bsn_kernels.hip", "bsn_poisson.hip", "bsn_api.cpp"]
 ARCH = os.environ.get("BSN_OFFLOAD_ARCH", "gfx950")
 
 
@@ -32,7 +32,8 @@ def build(verbose: bool = False) -> str:
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-Wall", "-Wno-unused-function", "-I", CSRC, "-I", os.path.join(HERE, "..", "include"),
-           "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+           "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES] + [
+           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lhipfft"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -21,6 +21,7 @@
 #include <vector>
 #include "bsn_internal.hpp"
 #include "bsn_launch.hpp"
+#include "bsn_poisson.hpp"
 #include "../../include/bsn.h"
 
 using namespace bsn;
@@ -79,6 +80,9 @@ struct bsn_grid {
     double* d_work = nullptr;      // third state buffer of the out-of-place step
     SrcDev hs{};                   // point masses and sinks (host copy)
     SrcDev* d_src = nullptr;
+    PoissonCtx* pois = nullptr;    // FFT Poisson solver (created on first use)
+    double* d_rho_in = nullptr;    // density staging for bsn_solve_poisson_isolated
+    bool self_grav = false;        // phi solved from each stage's density
     Plan* plan = nullptr;          // single-grid plan
 };
 
@@ -399,6 +403,12 @@ static int plan_step(Plan& P, int fused_cfl) {
     for (int s = 1; s <= 3; ++s) {
         const GridDev* T = P.d_st[s - 1];
         CHK(plan_fluxes(P, T, 2));
+        for (auto* g : P.grids) {
+            if (!g->self_grav) continue;
+            GridDev in = g->h;
+            in.U = s == 1 ? g->h.U : s == 2 ? g->h.Un : g->d_work;
+            HIPCHK(poisson_solve_state(*g->pois, in, s == 1, g->d_phi, L.stream));
+        }
         prof_mark(P, 3);
         HIPCHK(launch_zupdate(L, P.prec, P.zshape, T, P.d_zt, P.nzt, s, fused_cfl && s == 3));
         prof_mark(P, 9);
@@ -418,6 +428,9 @@ static int plan_step_inplace(Plan& P, int fused_cfl) {
             CHK(plan_ghost_fill(P, 0));
         }
         CHK(plan_fluxes(P));
+        for (auto* g : P.grids)
+            if (g->self_grav)
+                HIPCHK(poisson_solve_state(*g->pois, g->h, 0, g->d_phi, P.ctx->stream));
         CHK(plan_update(P, s, fused_cfl && s == 3));
     }
     CHK(plan_ghost_combine(P, 3));
@@ -554,6 +567,8 @@ extern "C" int bsn_grid_destroy(bsn_grid* g) {
     (void)hipFree(g->d_part);
     (void)hipFree(g->d_work);
     (void)hipFree(g->d_src);
+    if (g->pois) { poisson_destroy(*g->pois); delete g->pois; }
+    (void)hipFree(g->d_rho_in);
     delete g;
     return BSN_OK;
 }
@@ -754,6 +769,65 @@ extern "C" int bsn_add_sink_sources(bsn_grid* g, double* dU, int where, double d
     return add_src(g, dU, where, dx, dy, dz, S, 1);
 }
 
+static int ensure_poisson(bsn_grid* g, double dx, double dy, double dz) {
+    PoissonCtx* P = g->pois;
+    if (P && P->dx == dx && P->dy == dy && P->dz == dz) return BSN_OK;
+    HIPCHK(hipStreamSynchronize(g->ctx->stream));
+    if (P) poisson_destroy(*P);
+    else P = g->pois = new PoissonCtx();
+    hipError_t e = poisson_create(*P, g->h.nx, g->h.ny, g->h.nz, dx, dy, dz, g->ctx->stream,
+                                  g->ctx->cus);
+    if (e != hipSuccess) {
+        poisson_destroy(*P);
+        delete P;
+        g->pois = nullptr;
+        return set_err(BSN_ERR_HIP, "Poisson solver set-up (hipFFT): %s", hipGetErrorString(e));
+    }
+    return BSN_OK;
+}
+
+static int ensure_phi(bsn_grid* g) {
+    if (!g->d_phi)
+        HIPCHK(hipMalloc((void**)&g->d_phi, (size_t)g->h.nx * g->h.ny * g->h.nz * 8));
+    return BSN_OK;
+}
+
+extern "C" int bsn_solve_poisson_isolated(bsn_grid* g, const double* rho, int where_rho,
+                                          double* phi, int where_phi, double dx, double dy,
+                                          double dz) {
+    if (!g || !rho || !phi) return set_err(BSN_ERR_ARG, "NULL argument");
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0)) return set_err(BSN_ERR_ARG, "cell sizes must be > 0");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    CHK(ensure_poisson(g, dx, dy, dz));
+    const size_t n = (size_t)g->h.nx * g->h.ny * g->h.nz;
+    // staging: density in, potential out (host arrays only)
+    if (!g->d_rho_in) HIPCHK(hipMalloc((void**)&g->d_rho_in, 2 * n * 8));
+    const double* drho = where_rho == BSN_HOST ? g->d_rho_in : rho;
+    double* dphi = where_phi == BSN_HOST ? g->d_rho_in + n : phi;
+    if (where_rho == BSN_HOST)
+        HIPCHK(hipMemcpyAsync(g->d_rho_in, rho, n * 8, hipMemcpyHostToDevice, g->ctx->stream));
+    HIPCHK(poisson_solve_array(*g->pois, drho, dphi, g->ctx->stream));
+    if (where_phi == BSN_HOST)
+        HIPCHK(hipMemcpyAsync(phi, dphi, n * 8, hipMemcpyDeviceToHost, g->ctx->stream));
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_grid_set_self_gravity(bsn_grid* g, int on) {
+    if (!g || (on != 0 && on != 1)) return set_err(BSN_ERR_ARG, "bad arguments");
+    if (on && !g->params_set) return set_err(BSN_ERR_STATE, "grid parameters not set");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    HIPCHK(hipStreamSynchronize(g->ctx->stream));
+    if (on) {
+        CHK(ensure_poisson(g, g->dx, g->dy, g->dz));
+        CHK(ensure_phi(g));
+        g->h.phi = g->d_phi;
+    } else if (g->self_grav) {
+        g->h.phi = nullptr;
+    }
+    g->self_grav = on;
+    return grid_changed(g);
+}
+
 extern "C" int bsn_grid_set_phi(bsn_grid* g, const double* phi, int where) {
     if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
     HIPCHK(hipSetDevice(g->ctx->device));
@@ -800,6 +874,7 @@ static int apply_params(bsn_grid* g, double dx, double dy, double dz, double gam
     if (!same) {
         CHK(grid_changed(g));
         if (g->hs.nbh > 0 || g->hs.nsink > 0) CHK(src_upload(g));
+        if (g->self_grav) CHK(ensure_poisson(g, dx, dy, dz));
     }
     return BSN_OK;
 }
@@ -1125,7 +1200,9 @@ static int plan_run(Plan& P, double cfl, double t_end, int nsteps, int use_graph
     const LaunchCtx L = P.ctx->L();
     for (auto* g : P.grids) HIPCHK(launch_set_scal(L, g->h.scal, SCAL_T, 0.0));
     CHK(plan_cfl(P, cfl, t_end));
-    if (!use_graph) {
+    bool fft = false;
+    for (auto* g : P.grids) fft = fft || g->self_grav;
+    if (!use_graph || fft) {     // FFT plans are launched eagerly
         for (int s = 0; s < nsteps; ++s) CHK(plan_device_step(P));
         return sync_ctx(P.ctx);
     }

@@ -135,6 +135,20 @@ int bsn_grid_set_sinks(bsn_grid* g, int n, const double* pos, const double* vel,
                        const double* r_sink, const double* t_sink, int torque_free,
                        double rho_sink_min, double x0, double y0, double z0);
 
+/* Gas self-gravity ---------------------------------------------------------- */
+/* solve_poisson_isolated(rho, nx, ny, nz, dx, dy, dz) -> phi
+ * (src/self_gravity.jl:35-76): Hockney-Eastwood zero-padded FFT convolution
+ * with the softened kernel -1/sqrt(r^2 + eps^2), eps^2 = (dx dy dz)^(2/3).  rho
+ * and phi are (nx, ny, nz) column-major arrays of the grid's active shape. */
+int bsn_solve_poisson_isolated(bsn_grid* g, const double* rho, int where_rho, double* phi,
+                               int where_phi, double dx, double dy, double dz);
+/* on = 1: every later step solves phi from the density of each stage's input
+ * (after its floors) and adds the self-gravity source with it, as the
+ * reference's coupled step does with add_self_gravity_source!
+ * (src/self_gravity.jl:93-115).  It replaces a phi set by bsn_grid_set_phi;
+ * on = 0 removes it.  Uniform grids and batches; FMR levels keep their own phi. */
+int bsn_grid_set_self_gravity(bsn_grid* g, int on);
+
 /* Host-array convenience drop-ins: the reference signature on a host Julia
  * Array U; a cached per-thread grid uploads, runs and downloads. */
 int bsn_euler3d_step_host(double* U, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
