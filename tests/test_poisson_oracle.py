@@ -59,3 +59,20 @@ def test_symmetric_source_has_no_net_momentum():
     O.apply_sg_force(dU, U, phi, dx, dx, dx)
     s = dU[NG:-NG, NG:-NG, NG:-NG, 1:4].sum((0, 1, 2))
     assert np.abs(s).max() < 1e-10 * np.abs(dU[..., 1:4]).max()
+
+
+def test_equals_direct_sum_on_a_ragged_grid():
+    # The zero-padded (2n) circular convolution is the linear convolution
+    # exactly: phi_i = dV * sum_j rho_j * (-1 / sqrt(|x_i - x_j|^2 + eps^2)),
+    # eps^2 = (dx dy dz)^(2/3) (src/self_gravity.jl:44-45), by brute force.
+    # A dropped fold, a transposed axis or a wrong scale fails this.
+    nx, ny, nz = 7, 5, 4
+    dx, dy, dz = 0.1, 0.13, 0.07
+    rho = np.random.default_rng(5).uniform(0.0, 2.0, (nz, ny, nx))
+    eps2 = (dx * dy * dz) ** (2.0 / 3.0)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    X, Y, Z = (i * dx).ravel(), (j * dy).ravel(), (k * dz).ravel()
+    r2 = (X[:, None] - X[None]) ** 2 + (Y[:, None] - Y[None]) ** 2 + (Z[:, None] - Z[None]) ** 2
+    direct = (-1.0 / np.sqrt(r2 + eps2)) @ rho.ravel() * (dx * dy * dz)
+    phi = PO.solve_poisson_isolated(rho, dx, dy, dz)
+    assert np.abs(phi.ravel() - direct).max() <= 1e-13 * np.abs(direct).max()
