@@ -3,7 +3,8 @@
 # The drop-in for the reference's uniform-grid and FMR entry points on AMD
 # MI355X.  Methods keep the reference's names, positional arguments and
 # keywords (src/euler3d.jl:33-260, src/rk3.jl:14, src/fmr3d.jl:451,616,
-# src/fmr_gravity.jl:146):
+# src/fmr_gravity.jl:146, src/gravity_bh.jl:58, src/sinks.jl:32,
+# src/self_gravity.jl:35):
 #
 #   * on a `HIPGrid` (state resident in GPU memory, the analogue of a
 #     CuArray U) every call runs on the device without host transfers;
@@ -16,7 +17,9 @@ module BinarySupernovaHIP
 
 export HIPGrid, euler3d_step!, euler3d_rhs!, cfl_dt_3d, apply_floors_3d!,
        fill_ghost_3d_outflow!, fill_ghost_3d_periodic!, rk3_step!, HIPFMR,
-       fmr3d_step!, cfl_dt_fmr3d, set_phi!, download
+       fmr3d_step!, cfl_dt_fmr3d, set_phi!, download, add_bh_gravity_source!,
+       add_sink_sources!, set_point_masses!, set_sinks!, solve_poisson_isolated,
+       set_self_gravity!
 
 const libbsn = joinpath(@__DIR__, "..", "libbsn_hip.so")
 const NG = 3
@@ -209,6 +212,84 @@ function cfl_dt_fmr3d(G::HIPFMR, cfl::Real)
     check(ccall((:bsn_cfl_dt_fmr3d, libbsn), Cint, (Ptr{Cvoid}, Float64, Ref{Float64}),
                 G.handle, cfl, dt))
     return dt[]
+end
+
+# ---------------------------------------------------------------------------
+# Point masses and sinks (src/gravity_bh.jl:58-78, src/sinks.jl:32-58).
+# `bhs` is the reference's Vector{BlackHole} (fields pos, vel, mass, eps);
+# r_sink and t_sink are the reference's functions (src/BinarySupernova.jl:159-171),
+# passed in so that this module does not depend on BinarySupernova.
+
+bh_pos(bhs) = Float64[c for bh in bhs for c in bh.pos[1:3]]
+bh_vel(bhs) = Float64[c for bh in bhs for c in bh.vel[1:3]]
+
+function add_bh_gravity_source!(dU::Array{Float64,4}, U::HIPGrid, nx::Int, ny::Int, nz::Int,
+                                dx::Real, dy::Real, dz::Real, bhs, x0::Real, y0::Real, z0::Real)
+    check(ccall((:bsn_add_bh_gravity_source, libbsn), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cint, Float64, Float64, Float64, Cint, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64),
+                U.handle, dU, BSN_HOST, dx, dy, dz, length(bhs), bh_pos(bhs),
+                Float64[bh.mass for bh in bhs], Float64[bh.eps for bh in bhs], x0, y0, z0))
+    return dU
+end
+
+function add_sink_sources!(dU::Array{Float64,4}, U::HIPGrid, nx::Int, ny::Int, nz::Int,
+                           dx::Real, dy::Real, dz::Real, bhs, x0::Real, y0::Real, z0::Real,
+                           r_sink, t_sink; f_sink::Float64 = 1.0, torque_free::Bool = true,
+                           ρ_sink_min::Float64 = 0.0)
+    check(ccall((:bsn_add_sink_sources, libbsn), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cint, Float64, Float64, Float64, Cint, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Float64, Float64, Float64,
+                 Float64),
+                U.handle, dU, BSN_HOST, dx, dy, dz, length(bhs), bh_pos(bhs), bh_vel(bhs),
+                Float64[r_sink(bh) for bh in bhs], Float64[t_sink(bh, f_sink) for bh in bhs],
+                Cint(torque_free), ρ_sink_min, x0, y0, z0))
+    return dU
+end
+
+"""Point masses every later step adds to each stage's RHS (coupled step order:
+divergence, point masses, phi, sinks).  An empty `bhs` removes them."""
+function set_point_masses!(g::HIPGrid, bhs, x0::Real, y0::Real, z0::Real)
+    check(ccall((:bsn_grid_set_point_masses, libbsn), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64,
+                 Float64),
+                g.handle, length(bhs), bh_pos(bhs), Float64[bh.mass for bh in bhs],
+                Float64[bh.eps for bh in bhs], x0, y0, z0))
+    return g
+end
+
+function set_sinks!(g::HIPGrid, bhs, x0::Real, y0::Real, z0::Real, r_sink, t_sink;
+                    f_sink::Float64 = 1.0, torque_free::Bool = true, ρ_sink_min::Float64 = 0.0)
+    check(ccall((:bsn_grid_set_sinks, libbsn), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint,
+                 Float64, Float64, Float64, Float64),
+                g.handle, length(bhs), bh_pos(bhs), bh_vel(bhs),
+                Float64[r_sink(bh) for bh in bhs], Float64[t_sink(bh, f_sink) for bh in bhs],
+                Cint(torque_free), ρ_sink_min, x0, y0, z0))
+    return g
+end
+
+# ---------------------------------------------------------------------------
+# Self-gravity (src/self_gravity.jl:35-76)
+
+"""solve_poisson_isolated(ρ, nx, ny, nz, dx, dy, dz) on the GPU (hipFFT).  The
+grid `g` (of active shape nx, ny, nz) holds the cached plans and spectrum."""
+function solve_poisson_isolated(g::HIPGrid, ρ::Array{Float64,3}, nx::Int, ny::Int, nz::Int,
+                                dx::Real, dy::Real, dz::Real)
+    size(ρ) == (nx, ny, nz) || throw(DimensionMismatch("ρ must be (nx, ny, nz)"))
+    Φ = similar(ρ)
+    check(ccall((:bsn_solve_poisson_isolated, libbsn), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Float64, Float64, Float64),
+                g.handle, ρ, BSN_HOST, Φ, BSN_HOST, dx, dy, dz))
+    return Φ
+end
+
+"""Every later step solves phi from each RK3 stage's density and adds the
+self-gravity source (the coupled step with self_gravity on)."""
+function set_self_gravity!(g::HIPGrid, on::Bool = true)
+    check(ccall((:bsn_grid_set_self_gravity, libbsn), Cint, (Ptr{Cvoid}, Cint), g.handle,
+                Cint(on)))
+    return g
 end
 
 end # module

@@ -34,3 +34,28 @@ def test_version_and_error_string_without_gpu():
 def test_bc_masks():
     assert bsn.bc_mask("outflow") == 0 and bsn.bc_mask("periodic") == 7
     assert bsn.bc_mask(["outflow", "periodic", "periodic"]) == 6
+
+
+def header_arg_counts():
+    text = (ROOT / "include" / "bsn.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for name, args in re.findall(r"\b(bsn_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+        args = args.strip()
+        out[name] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_julia_ccalls_match_the_header():
+    # every ccall of the Julia binding names a declared function with the
+    # declared number of arguments (the binding cannot be run here)
+    src = (ROOT / "binary-supernova_amd" / "julia" / "BinarySupernovaHIP.jl").read_text()
+    decl = header_arg_counts()
+    calls = re.findall(r"ccall\(\(:(bsn_[a-z0-9_]+),\s*libbsn\),\s*\w+,\s*\(([^()]*(?:\([^()]*\)[^()]*)*)\)",
+                       src, flags=re.S)
+    assert len(calls) >= 20
+    for name, types in calls:
+        assert name in decl, name
+        t = types.strip().rstrip(",")
+        n = 0 if not t else t.count(",") + 1
+        assert n == decl[name], (name, n, decl[name])
