@@ -13,7 +13,8 @@ computed on the device each step.  The timed region is the split form; the
 FP64 form is run and timed the same way for the speed-up and the L1 error.
 
 Other workloads (--config): sod128, sedov128, batch48, fmr3, selfgrav128 (the
-coupled step with an FFT Poisson solve every RK3 stage).
+coupled step with an FFT Poisson solve every RK3 stage), binary128 (the coupled
+step with two bodies: gas forces, sinks, point masses, self-gravity, accretion).
 
 Prints ONE JSON line.  Inputs are resident in HBM before the timed region.
 """
@@ -523,13 +524,108 @@ def bench_selfgrav(bsn, args):
     return out
 
 
+def bench_binary(bsn, args):
+    """The coupled step of the binary run (coupled_step!,
+    scripts/run_sn50_fiducial.jl:756-819) with every gas operation on the
+    device: a gamma = 5/3 polytrope star on n^3 over [-2, 2]^3 with two bodies
+    (one at the centre, one at x = 1.2 on a circular orbit), torque-free
+    sinks, point-mass gravity and self-gravity in every RK3 stage.  Per step:
+    gas_force_on_bh for both bodies, a kick-drift of the bodies on the host
+    (the N-body integrator is not part of this build), the step, accrete! for
+    both bodies.  The gas diagnostics run once per step as well."""
+    import copy
+    n = args.n or 128
+    gamma, L = 5.0 / 3.0, 4.0
+    d = (L / n,) * 3
+    o = (-L / 2,) * 3
+    floors = (1e-10, 1e-12)
+    U = bsn.problems.polytrope_ic_3d(n, n, n, *d, gamma, x0=o[0], y0=o[1], z0=o[2],
+                                     rho_floor=floors[0], P_floor=floors[1])
+    B = bsn.BlackHole
+    bhs0 = [B(pos=[0.0, 0.0, 0.0], vel=[0.0, 0.0, 0.0], mass=0.5, eps=d[0], c_code=1e3,
+              r_floor=2.5 * d[0]),
+            B(pos=[1.2, 0.0, 0.0], vel=[0.0, 0.9, 0.0], mass=0.3, eps=d[0], c_code=1e3,
+              r_floor=2.5 * d[0])]
+    steps = args.steps
+    g = bsn.Grid(n, n, n, "split").set_state(U)
+    g.set_params(*d, gamma, "outflow", *floors)
+    b = bsn.Batch([g])
+    b.autotune()
+    bsn.set_self_gravity(g, True)
+    dt = 0.2 * bsn.cfl_dt_3d(g, n, n, n, *d, gamma, 0.4)
+
+    def run(k, timed):
+        bhs = copy.deepcopy(bhs0)
+        g.set_state(U)
+        t = dict(force=0.0, step=0.0, accrete=0.0, diag=0.0)
+        g.ctx.synchronize()
+        t_all = time.perf_counter()
+        for _ in range(k):
+            t0 = time.perf_counter()
+            F = bsn.gas_forces_on_bhs(g, n, n, n, *d, bhs, *o)
+            t1 = time.perf_counter()
+            for bh, f in zip(bhs, F):              # kick-drift with the gas force only
+                for c in range(3):
+                    bh.vel[c] += dt * f[c] / bh.mass
+                    bh.pos[c] += dt * bh.vel[c]
+            bsn.set_point_masses(g, bhs, *o)
+            bsn.set_sinks(g, bhs, *o, f_sink=1.0, torque_free=True, rho_sink_min=1e-6)
+            t2 = time.perf_counter()
+            b.step([dt])
+            g.ctx.synchronize()
+            t3 = time.perf_counter()
+            bsn.accrete_all_(bhs, g, n, n, n, *d, *o, dt, rho_sink_min=1e-6)
+            t4 = time.perf_counter()
+            bsn.diagnostics.gas_diagnostics(g, n, n, n, *d, *o, bhs, gamma)
+            t5 = time.perf_counter()
+            t["force"] += t1 - t0
+            t["step"] += t3 - t2
+            t["accrete"] += t4 - t3
+            t["diag"] += t5 - t4
+        wall = time.perf_counter() - t_all
+        return wall, t, bhs
+
+    run(args.warmup, False)
+    wall, t, bhs = run(steps, True)
+    log(f"binary {n}^3: {1e3 * wall / steps:.3f} ms per coupled step")
+    cells = n ** 3
+    out = dict(
+        metric="zone-updates/s per coupled RK3 step (point masses, sinks, self-gravity, "
+               "gas forces, accretion; split FP32/FP64)",
+        value=cells * steps / wall, unit="zone-updates/s",
+        ms_per_step=1e3 * wall / steps, dtype="f32-flux/f64-state",
+        config=dict(workload=f"gamma 5/3 polytrope star on {n}^3 over [-2,2]^3 with two bodies, "
+                             f"torque-free sinks, point-mass gravity and self-gravity every RK3 "
+                             f"stage, gas forces and accretion per step, fixed dt",
+                    grid=[n, n, n], precision="split", bodies=2),
+        ms_per_call={k: 1e3 * v / steps for k, v in t.items()},
+        accreted_mass=float(sum(bh.mass for bh in bhs) - sum(bh.mass for bh in bhs0)),
+    )
+    if not args.no_cpu:
+        log("cpu baseline (the host reductions of the reference: 2 gas forces, 2 accretions)")
+        sys.path.insert(0, str(ROOT))
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        for bh in bhs0:
+            O.gas_force_on_bh(U, *d, bh.pos, bh.mass, bh.eps, *o)
+            O.accrete_sums(U, *d, *o, bh.pos, bsn.r_sink(bh), bsn.t_sink(bh), dt, 1e-6)
+        sec = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(
+            value=1e3 * sec, unit="ms per step (gas forces + accretion sums, 2 bodies)", cores=1,
+            kind="port", sample=f"gas_force_on_bh and accrete! sums of both bodies on the same "
+                                f"{n}^3 state, sequential FP64 C restatement, {sec:.2f} s")
+        out["gpu_ms_force_and_accrete"] = out["ms_per_call"]["force"] + out["ms_per_call"]["accrete"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="turb256",
-                    choices=["turb256", "sedov128", "sod128", "batch48", "fmr3", "selfgrav128"])
+                    choices=["turb256", "sedov128", "sod128", "batch48", "fmr3", "selfgrav128",
+                             "binary128"])
     ap.add_argument("--n", type=int, default=None, help="override the grid size")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lean", action="store_true", help="skip the hipGraph replay pass "
@@ -559,6 +655,9 @@ def main():
     elif args.config == "selfgrav128":
         args.steps = args.steps or 20
         out = bench_selfgrav(bsn, args)
+    elif args.config == "binary128":
+        args.steps = args.steps or 20
+        out = bench_binary(bsn, args)
     else:
         args.steps = args.steps or 100
         out = bench_sod(bsn, args)

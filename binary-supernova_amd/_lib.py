@@ -25,7 +25,8 @@ EXPORTS = [
     "bsn_cfl_dt_3d", "bsn_apply_sg_force", "bsn_rk3_save", "bsn_rk3_combine",
     "bsn_conserved_totals", "bsn_add_bh_gravity_source", "bsn_add_sink_sources",
     "bsn_grid_set_point_masses", "bsn_grid_set_sinks", "bsn_solve_poisson_isolated",
-    "bsn_grid_set_self_gravity", "bsn_euler3d_step_host", "bsn_cfl_dt_3d_host", "bsn_batch_create",
+    "bsn_grid_set_self_gravity", "bsn_gas_force_on_bh", "bsn_accrete", "bsn_gas_diagnostics",
+    "bsn_euler3d_step_host", "bsn_cfl_dt_3d_host", "bsn_batch_create",
     "bsn_batch_destroy", "bsn_batch_autotune", "bsn_batch_set_shapes", "bsn_batch_set_mode",
     "bsn_batch_set_update_shape", "bsn_batch_step",
     "bsn_batch_run", "bsn_batch_times", "bsn_fmr_create", "bsn_fmr_destroy",
@@ -81,6 +82,9 @@ def load():
         "bsn_grid_set_sinks": ([_p, _i, _pd, _pd, _pd, _pd, _i, _d, _d, _d, _d], _i),
         "bsn_solve_poisson_isolated": ([_p, _p, _i, _p, _i, _d, _d, _d], _i),
         "bsn_grid_set_self_gravity": ([_p, _i], _i),
+        "bsn_gas_force_on_bh": ([_p, _d, _d, _d, _i, _p, _p, _p, _d, _d, _d, _p], _i),
+        "bsn_accrete": ([_p, _d, _d, _d, _i, _p, _p, _p, _p, _p, _d, _d, _d, _d, _d, _p], _i),
+        "bsn_gas_diagnostics": ([_p, _d, _d, _d, _d, _d, _d, _i, _p, _p, _p, _d, _p], _i),
         "bsn_euler3d_step_host": ([_p, _i64, _i64, _i64, _d, _d, _d, _d, _d, _i, _d, _d, _i], _i),
         "bsn_cfl_dt_3d_host": ([_p, _i64, _i64, _i64, _d, _d, _d, _d, _d, _pd], _i),
         "bsn_batch_create": ([_p, _i, C.POINTER(_p), C.POINTER(_p)], _i),

@@ -16,7 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsn_hip.so")
-SOURCES = ["bsn_kernels.hip", "bsn_poisson.hip", "bsn_api.cpp"]
+SOURCES = ["bsn_kernels.hip", "bsn_poisson.hip", "bsn_bodies.hip", "bsn_api.cpp"]
 ARCH = os.environ.get("BSN_OFFLOAD_ARCH", "gfx950")
 
 

@@ -22,6 +22,7 @@
 #include "bsn_internal.hpp"
 #include "bsn_launch.hpp"
 #include "bsn_poisson.hpp"
+#include "bsn_bodies.hpp"
 #include "../../include/bsn.h"
 
 using namespace bsn;
@@ -83,6 +84,7 @@ struct bsn_grid {
     PoissonCtx* pois = nullptr;    // FFT Poisson solver (created on first use)
     double* d_rho_in = nullptr;    // density staging for bsn_solve_poisson_isolated
     bool self_grav = false;        // phi solved from each stage's density
+    double* d_body = nullptr;      // body/diagnostic reductions: partial sums, then the sums
     Plan* plan = nullptr;          // single-grid plan
 };
 
@@ -569,6 +571,7 @@ extern "C" int bsn_grid_destroy(bsn_grid* g) {
     (void)hipFree(g->d_src);
     if (g->pois) { poisson_destroy(*g->pois); delete g->pois; }
     (void)hipFree(g->d_rho_in);
+    (void)hipFree(g->d_body);
     delete g;
     return BSN_OK;
 }
@@ -1017,6 +1020,119 @@ extern "C" int bsn_conserved_totals(bsn_grid* g, double dV, double out[5]) {
     return sync_ctx(g->ctx);
 }
 
+// ---------------------------------------------------------------------------
+// What the bodies and the diagnostics read from the gas (bsn_bodies.hip)
+
+#define BODY_OUT BQ_PART_DOUBLES   // the sums follow the partials
+
+static int body_begin(bsn_grid* g, BodyQ& Q, double dx, double dy, double dz, double x0,
+                      double y0, double z0, int n, const double* pos) {
+    if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0)) return set_err(BSN_ERR_ARG, "cell sizes must be > 0");
+    if (n < 0 || n > BSN_MAX_BH) return set_err(BSN_ERR_ARG, "body count %d (max %d)", n, BSN_MAX_BH);
+    if (n > 0 && !pos) return set_err(BSN_ERR_ARG, "NULL position array");
+    Q = BodyQ{};
+    Q.n = n;
+    Q.x0 = x0; Q.y0 = y0; Q.z0 = z0;
+    Q.dx = dx; Q.dy = dy; Q.dz = dz;
+    Q.dV = dx * dy * dz;
+    for (int b = 0; b < n; ++b) {
+        Q.px[b] = pos[3 * b]; Q.py[b] = pos[3 * b + 1]; Q.pz[b] = pos[3 * b + 2];
+    }
+    HIPCHK(hipSetDevice(g->ctx->device));
+    if (!g->d_body) HIPCHK(hipMalloc((void**)&g->d_body, (BODY_OUT + 64) * sizeof(double)));
+    return BSN_OK;
+}
+
+static int body_finish(bsn_grid* g, hipError_t e, double* host, int count) {
+    if (e != hipSuccess) return set_err(BSN_ERR_HIP, "launch: %s", hipGetErrorString(e));
+    if (count > 0)
+        HIPCHK(hipMemcpyAsync(host, g->d_body + BODY_OUT, count * sizeof(double),
+                              hipMemcpyDeviceToHost, g->ctx->stream));
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_gas_force_on_bh(bsn_grid* g, double dx, double dy, double dz, int n,
+                                   const double* pos, const double* mass, const double* eps,
+                                   double x0, double y0, double z0, double* F) {
+    BodyQ Q;
+    CHK(body_begin(g, Q, dx, dy, dz, x0, y0, z0, n, pos));
+    if (n > 0 && (!mass || !eps || !F)) return set_err(BSN_ERR_ARG, "NULL argument");
+    if (n == 0) return BSN_OK;
+    for (int b = 0; b < n; ++b) { Q.mass[b] = mass[b]; Q.eps[b] = eps[b]; }
+    hipError_t e = launch_body_force(g->ctx->stream, g->ctx->cus, g->h, Q, g->d_body,
+                                     g->d_body + BODY_OUT);
+    return body_finish(g, e, F, 3 * n);
+}
+
+// Active cells (0-based first, extent) along one axis whose centres
+// o + (ii - 0.5) d can lie within rs of p, one cell of margin each side.  A
+// NaN or infinite position or radius makes the reference's `r >= rs` test
+// false for every cell or leaves it undecided here, so the whole axis is kept;
+// rs <= 0 excludes every cell (r >= 0 >= rs).
+static void sink_axis(double p, double rs, double o, double d, int nact, int& lo, int& ext) {
+    if (!std::isfinite(p) || !std::isfinite(rs)) { lo = 0; ext = nact; return; }
+    if (!(rs > 0.0)) { lo = 0; ext = 0; return; }
+    double a = std::floor((p - rs - o) / d + 0.5) - 1.0;      // 1-based
+    double b = std::ceil((p + rs - o) / d + 0.5) + 1.0;
+    if (!(a >= 1.0)) a = 1.0;
+    if (!(b <= (double)nact)) b = (double)nact;
+    if (b < a) { lo = 0; ext = 0; return; }
+    lo = (int)a - 1;
+    ext = (int)b - (int)a + 1;
+}
+
+extern "C" int bsn_accrete(bsn_grid* g, double dx, double dy, double dz, int n, const double* pos,
+                           double* vel, double* mass, const double* r_sink, const double* t_sink,
+                           double rho_sink_min, double x0, double y0, double z0, double dt,
+                           double* dm_out) {
+    BodyQ Q;
+    CHK(body_begin(g, Q, dx, dy, dz, x0, y0, z0, n, pos));
+    if (n > 0 && (!vel || !mass || !r_sink || !t_sink)) return set_err(BSN_ERR_ARG, "NULL argument");
+    if (n == 0) return BSN_OK;
+    Q.rho_sink_min = rho_sink_min;
+    const int nact[3] = {g->h.nx, g->h.ny, g->h.nz};
+    const double o[3] = {x0, y0, z0}, dd[3] = {dx, dy, dz};
+    for (int b = 0; b < n; ++b) {
+        Q.rs[b] = r_sink[b];
+        Q.rate[b] = Q.dV / t_sink[b] * dt;       // rate = dV / ts * dt (src/sinks.jl:97)
+        const double p[3] = {Q.px[b], Q.py[b], Q.pz[b]};
+        for (int a = 0; a < 3; ++a) sink_axis(p[a], r_sink[b], o[a], dd[a], nact[a], Q.lo[b][a], Q.ext[b][a]);
+    }
+    double s[4 * BSN_MAX_BH];
+    hipError_t e = launch_body_accrete(g->ctx->stream, g->ctx->cus, g->h, Q, g->d_body,
+                                       g->d_body + BODY_OUT);
+    CHK(body_finish(g, e, s, 4 * n));
+    // the body update of accrete! (src/sinks.jl:104-108)
+    for (int b = 0; b < n; ++b) {
+        const double dm = s[4 * b];
+        if (dm_out) dm_out[b] = dm;
+        if (dm > 0.0) {
+            const double M_new = mass[b] + dm;
+            for (int c = 0; c < 3; ++c) vel[3 * b + c] = (mass[b] * vel[3 * b + c] + s[4 * b + 1 + c]) / M_new;
+            mass[b] = M_new;
+        }
+    }
+    return BSN_OK;
+}
+
+extern "C" int bsn_gas_diagnostics(bsn_grid* g, double dx, double dy, double dz, double x0,
+                                   double y0, double z0, int n, const double* pos,
+                                   const double* mass, const double* eps, double gamma,
+                                   double out[9]) {
+    BodyQ Q;
+    CHK(body_begin(g, Q, dx, dy, dz, x0, y0, z0, n, pos));
+    if (!out || (n > 0 && (!mass || !eps))) return set_err(BSN_ERR_ARG, "NULL argument");
+    for (int b = 0; b < n; ++b) { Q.mass[b] = mass[b]; Q.eps[b] = eps[b]; }
+    Q.gamma = gamma;
+    double s[BQ_NACC_DIAG];
+    hipError_t e = launch_body_diag(g->ctx->stream, g->ctx->cus, g->h, Q, g->d_body,
+                                    g->d_body + BODY_OUT);
+    CHK(body_finish(g, e, s, BQ_NACC_DIAG));
+    for (int q = 0; q < BQ_NACC_DIAG; ++q) out[q] = s[q] * Q.dV;   // Float64(sum(...)) * dV
+    return BSN_OK;
+}
+
 // Host-array drop-ins with a per-thread cached context and grid.
 static thread_local bsn_ctx* t_ctx = nullptr;
 static thread_local bsn_grid* t_grid = nullptr;
@@ -1098,6 +1214,7 @@ extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
     if (!b) return set_err(BSN_ERR_ARG, "NULL batch");
     Plan& P = *b->plan;
     HIPCHK(hipSetDevice(P.ctx->device));
+    CHK(plan_refresh(P));
     const LaunchCtx L = P.ctx->L();
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -1245,6 +1362,7 @@ extern "C" int bsn_batch_profile(bsn_batch* b, int nsteps, double* ms_flux3, dou
     if (!b || nsteps < 1) return set_err(BSN_ERR_ARG, "bad arguments");
     Plan& P = *b->plan;
     HIPCHK(hipSetDevice(P.ctx->device));
+    CHK(plan_refresh(P));
     const LaunchCtx L = P.ctx->L();
     hipEvent_t s0, s1;
     HIPCHK(hipEventCreate(&s0));

@@ -7,8 +7,12 @@ add_bh_gravity_source! (src/gravity_bh.jl:58-78) and add_sink_sources!
 attach the bodies to a grid so that every later step adds them to each
 stage's RHS inside the fused update kernel, in the order of the reference's
 coupled step (scripts/run_sn50_fiducial.jl:772-816): flux divergence,
-point-mass gravity, the phi source, sinks.  The N-body update and accrete!
-stay with the caller.
+point-mass gravity, the phi source, sinks.
+
+``gas_force_on_bh`` (src/gravity_bh.jl:91-113) and ``accrete_``
+(src/sinks.jl:73-109) read the gas on the device: only the sums come back to
+the host, where the reference copies the grid to the host every call.  The
+N-body update stays with the caller.
 """
 from __future__ import annotations
 
@@ -105,3 +109,54 @@ def set_sinks(grid: Grid, bhs, x0, y0, z0, *, f_sink=1.0, torque_free=True,
     pos, vel, rs, ts = _sinks(bhs, f_sink)
     call("bsn_grid_set_sinks", grid.handle, len(bhs), pos[1], vel[1], rs[1], ts[1],
          int(bool(torque_free)), float(rho_sink_min), float(x0), float(y0), float(z0))
+
+
+def gas_forces_on_bhs(U, nx, ny, nz, dx, dy, dz, bhs, x0, y0, z0) -> np.ndarray:
+    """gas_force_on_bh for every body of `bhs` in one pass over the density:
+    an (n, 3) array, row b the force of the gas on bhs[b]."""
+    g, _ = _as_grid(U, nx, ny, nz)
+    pos, mass, eps = _masses(bhs)
+    F = np.zeros((len(bhs), 3))
+    if len(bhs):
+        call("bsn_gas_force_on_bh", g.handle, float(dx), float(dy), float(dz), len(bhs), pos[1],
+             mass[1], eps[1], float(x0), float(y0), float(z0), _ptr(F))
+    return F
+
+
+def gas_force_on_bh(U, nx, ny, nz, dx, dy, dz, bh, x0, y0, z0) -> np.ndarray:
+    """gas_force_on_bh (src/gravity_bh.jl:91-113): F = +sum rho grad(Phi_bh) dV."""
+    return gas_forces_on_bhs(U, nx, ny, nz, dx, dy, dz, [bh], x0, y0, z0)[0]
+
+
+def accrete_all_(bhs, U, nx, ny, nz, dx, dy, dz, x0, y0, z0, dt, *, f_sink=1.0,
+                 rho_sink_min=0.0) -> np.ndarray:
+    """accrete! for every body of `bhs` (each with its own sink radius and
+    time, from the state U): updates bh.mass and bh.vel in place and returns
+    the accreted masses."""
+    g, _ = _as_grid(U, nx, ny, nz)
+    n = len(bhs)
+    if n > MAX_BODIES:
+        raise ValueError(f"at most {MAX_BODIES} bodies")
+    pos = _arr([c for b in bhs for c in b.pos])
+    vel = np.ascontiguousarray([float(c) for b in bhs for c in b.vel], dtype=np.float64)
+    mass = np.ascontiguousarray([float(b.mass) for b in bhs], dtype=np.float64)
+    rs = _arr([r_sink(b) for b in bhs])
+    ts = _arr([t_sink(b, f_sink) for b in bhs])
+    dm = np.zeros(n)
+    if n:
+        call("bsn_accrete", g.handle, float(dx), float(dy), float(dz), n, pos[1], _ptr(vel),
+             _ptr(mass), rs[1], ts[1], float(rho_sink_min), float(x0), float(y0), float(z0),
+             float(dt), _ptr(dm))
+    for b, bh in enumerate(bhs):
+        bh.mass = float(mass[b])
+        bh.vel = [float(v) for v in vel[3 * b:3 * b + 3]]
+    return dm
+
+
+def accrete_(bh, U, nx, ny, nz, dx, dy, dz, x0, y0, z0, dt, *, f_sink=1.0,
+             rho_sink_min=0.0) -> None:
+    """accrete! (src/sinks.jl:73-109): bh gains the mass and the full momentum
+    of the gas within r_sink(bh) drained over dt."""
+    accrete_all_([bh], U, nx, ny, nz, dx, dy, dz, x0, y0, z0, dt, f_sink=f_sink,
+                 rho_sink_min=rho_sink_min)
+

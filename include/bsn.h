@@ -149,6 +149,36 @@ int bsn_solve_poisson_isolated(bsn_grid* g, const double* rho, int where_rho, do
  * on = 0 removes it.  Uniform grids and batches; FMR levels keep their own phi. */
 int bsn_grid_set_self_gravity(bsn_grid* g, int on);
 
+/* What the bodies and the run's diagnostics read from the gas ---------------
+ * Device reductions over the grid's active cells; the host receives only the
+ * sums.  Bodies as above (pos: 3 values per body, n <= BSN_MAX_BH).  The
+ * per-cell terms follow the reference's operation order; the order of the sums
+ * differs (they are deterministic from call to call). */
+/* gas_force_on_bh(U, nx, ny, nz, dx, dy, dz, bh, x0, y0, z0) for n bodies in one
+ * pass over the density (src/gravity_bh.jl:91-113): F[3*b + c] is the force of
+ * the gas on body b, +sum rho M (r - r_b) / (|r - r_b|^2 + eps^2)^(3/2) dV. */
+int bsn_gas_force_on_bh(bsn_grid* g, double dx, double dy, double dz, int n, const double* pos,
+                        const double* mass, const double* eps, double x0, double y0, double z0,
+                        double* F);
+/* accrete!(bh, U, nx, ny, nz, dx, dy, dz, x0, y0, z0, dt; f_sink, rho_sink_min)
+ * (src/sinks.jl:73-109) for n bodies: the sums over the cells with r < r_sink[b]
+ * and rho > rho_sink_min run on the device over each body's sink box, then vel
+ * and mass (host arrays, updated in place) change as accrete! changes bh.vel and
+ * bh.mass when the accreted mass is > 0.  r_sink and t_sink are the caller's
+ * r_sink(bh) and t_sink(bh, f_sink).  dm_out (n values, may be NULL) receives the
+ * accreted masses. */
+int bsn_accrete(bsn_grid* g, double dx, double dy, double dz, int n, const double* pos,
+                double* vel, double* mass, const double* r_sink, const double* t_sink,
+                double rho_sink_min, double x0, double y0, double z0, double dt, double* dm_out);
+/* The gas diagnostics of src/diagnostics.jl in one pass over the state:
+ * out[0] gas_energy_total (:21-26), out[1] gas_kinetic_total (:33-44),
+ * out[2..4] gas_momentum_total (:51-59), out[5..7] gas_angular_momentum_total
+ * about the origin (:67-94), out[8] bound_gas_mass with the potential of the n
+ * bodies and gamma (:137-175). */
+int bsn_gas_diagnostics(bsn_grid* g, double dx, double dy, double dz, double x0, double y0,
+                        double z0, int n, const double* pos, const double* mass,
+                        const double* eps, double gamma, double out[9]);
+
 /* Host-array convenience drop-ins: the reference signature on a host Julia
  * Array U; a cached per-thread grid uploads, runs and downloads. */
 int bsn_euler3d_step_host(double* U, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,

@@ -690,6 +690,113 @@ int orc_coupled_step(double *U, int64_t nx, int64_t ny, int64_t nz,
 }
 
 /* ------------------------------------------------------------------------ */
+/* What the bodies and the diagnostics read from the gas.  Sequential sums in */
+/* the reference's loop order (k outer, i inner), one body per call as the    */
+/* reference calls them.                                                      */
+
+/* gas_force_on_bh (src/gravity_bh.jl:91-113) */
+void orc_gas_force_on_bh(const double *U, int64_t nx, int64_t ny, int64_t nz,
+                         double dx, double dy, double dz, double px, double py, double pz,
+                         double mass, double eps, double x0, double y0, double z0,
+                         double F[3])
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG;
+    const double dV = dx * dy * dz;
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    for (int64_t kk = 1; kk <= nz; ++kk)
+        for (int64_t jj = 1; jj <= ny; ++jj)
+            for (int64_t ii = 1; ii <= nx; ++ii) {
+                const double xc = x0 + ((double)ii - 0.5) * dx;
+                const double yc = y0 + ((double)jj - 0.5) * dy;
+                const double zc = z0 + ((double)kk - 0.5) * dz;
+                const double ddx = xc - px, ddy = yc - py, ddz = zc - pz;
+                const double r2 = ddx * ddx + ddy * ddy + ddz * ddz + eps * eps;
+                const double r3 = r2 * sqrt(r2);
+                const double rho = U[U_IDX(0, ii + NG - 1, jj + NG - 1, kk + NG - 1, Nx, Ny)];
+                const double fac = mass / r3 * rho * dV;
+                Fx += fac * ddx;
+                Fy += fac * ddy;
+                Fz += fac * ddz;
+            }
+    F[0] = Fx; F[1] = Fy; F[2] = Fz;
+}
+
+/* The sums of accrete! (src/sinks.jl:86-103): out = (dm, dPx, dPy, dPz). */
+void orc_accrete_sums(const double *U, int64_t nx, int64_t ny, int64_t nz,
+                      double dx, double dy, double dz, double x0, double y0, double z0,
+                      double px, double py, double pz, double rs, double ts, double dt,
+                      double rho_sink_min, double out[4])
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG;
+    const double dV = dx * dy * dz;
+    double dm = 0.0, dPx = 0.0, dPy = 0.0, dPz = 0.0;
+    for (int64_t kk = 1; kk <= nz; ++kk)
+        for (int64_t jj = 1; jj <= ny; ++jj)
+            for (int64_t ii = 1; ii <= nx; ++ii) {
+                const double xc = x0 + ((double)ii - 0.5) * dx;
+                const double yc = y0 + ((double)jj - 0.5) * dy;
+                const double zc = z0 + ((double)kk - 0.5) * dz;
+                const double r = sqrt((xc - px) * (xc - px) + (yc - py) * (yc - py) +
+                                      (zc - pz) * (zc - pz));
+                if (r >= rs) continue;
+                const int64_t i = ii + NG - 1, j = jj + NG - 1, k = kk + NG - 1;
+                if (U[U_IDX(0, i, j, k, Nx, Ny)] <= rho_sink_min) continue;
+                const double rate = dV / ts * dt;
+                dm += U[U_IDX(0, i, j, k, Nx, Ny)] * rate;
+                dPx += U[U_IDX(1, i, j, k, Nx, Ny)] * rate;
+                dPy += U[U_IDX(2, i, j, k, Nx, Ny)] * rate;
+                dPz += U[U_IDX(3, i, j, k, Nx, Ny)] * rate;
+            }
+    out[0] = dm; out[1] = dPx; out[2] = dPy; out[3] = dPz;
+}
+
+/* gas_energy_total, gas_kinetic_total, gas_momentum_total,
+ * gas_angular_momentum_total and bound_gas_mass (src/diagnostics.jl:21-175):
+ * out = (E, KE, Px, Py, Pz, Lx, Ly, Lz, M_bound), each sum times dV.  The
+ * reference sums with Julia's pairwise `sum`; these sums are sequential. */
+void orc_gas_diagnostics(const double *U, int64_t nx, int64_t ny, int64_t nz,
+                         double dx, double dy, double dz, double x0, double y0, double z0,
+                         int nbh, const double *px, const double *py, const double *pz,
+                         const double *mass, const double *eps, double g, double out[9])
+{
+    const int64_t Nx = nx + 2 * NG, Ny = ny + 2 * NG;
+    const double dV = dx * dy * dz;
+    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t kk = 1; kk <= nz; ++kk)
+        for (int64_t jj = 1; jj <= ny; ++jj)
+            for (int64_t ii = 1; ii <= nx; ++ii) {
+                const int64_t i = ii + NG - 1, j = jj + NG - 1, k = kk + NG - 1;
+                const double r = U[U_IDX(0, i, j, k, Nx, Ny)];
+                const double mx = U[U_IDX(1, i, j, k, Nx, Ny)];
+                const double my = U[U_IDX(2, i, j, k, Nx, Ny)];
+                const double mz = U[U_IDX(3, i, j, k, Nx, Ny)];
+                const double E = U[U_IDX(4, i, j, k, Nx, Ny)];
+                const double xc = x0 + ((double)ii - 0.5) * dx;
+                const double yc = y0 + ((double)jj - 0.5) * dy;
+                const double zc = z0 + ((double)kk - 0.5) * dz;
+                const double rs_ = jl_max(r, 1e-30);
+                const double KE = 0.5 * (mx * mx + my * my + mz * mz) / rs_;
+                s[0] += E;
+                s[1] += KE;
+                s[2] += mx;
+                s[3] += my;
+                s[4] += mz;
+                s[5] += yc * mz - zc * my;
+                s[6] += zc * mx - xc * mz;
+                s[7] += xc * my - yc * mx;
+                const double P = jl_max((g - 1.0) * (E - rs_ * KE), 0.0);
+                const double e_int = P / ((g - 1.0) * rs_);
+                double phi = 0.0;
+                for (int n = 0; n < nbh; ++n)
+                    phi -= mass[n] / sqrt((xc - px[n]) * (xc - px[n]) + (yc - py[n]) * (yc - py[n]) +
+                                          (zc - pz[n]) * (zc - pz[n]) + eps[n] * eps[n]);
+                const double e_tot = KE + e_int + phi;
+                s[8] += e_tot <= 0.0 ? rs_ : 0.0;
+            }
+    for (int q = 0; q < 9; ++q) out[q] = s[q] * dV;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Conservation totals (src/diagnostics.jl:21-58).  Sums run over active
  * cells in column-major order with pairwise blocks; the summation order of
  * Julia's sum is not reproduced, so these agree to rounding only.           */

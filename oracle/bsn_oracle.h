@@ -76,6 +76,18 @@ int orc_coupled_step(double *U, int64_t nx, int64_t ny, int64_t nz,
                      const double *vel, const double *mass, const double *eps,
                      int nsink, const double *rs, const double *ts, int torque_free,
                      double rho_sink_min);
+void orc_gas_force_on_bh(const double *U, int64_t nx, int64_t ny, int64_t nz,
+                         double dx, double dy, double dz, double px, double py, double pz,
+                         double mass, double eps, double x0, double y0, double z0,
+                         double F[3]);
+void orc_accrete_sums(const double *U, int64_t nx, int64_t ny, int64_t nz,
+                      double dx, double dy, double dz, double x0, double y0, double z0,
+                      double px, double py, double pz, double rs, double ts, double dt,
+                      double rho_sink_min, double out[4]);
+void orc_gas_diagnostics(const double *U, int64_t nx, int64_t ny, int64_t nz,
+                         double dx, double dy, double dz, double x0, double y0, double z0,
+                         int nbh, const double *px, const double *py, const double *pz,
+                         const double *mass, const double *eps, double g, double out[9]);
 
 #ifdef __cplusplus
 }

@@ -91,6 +91,10 @@ def _declare(L):
                                    _d, _d, _d, C.c_int, _pd, _pd, _pd, _pd, C.c_int, _pd, _pd,
                                    C.c_int, _d]
     L.orc_coupled_step.restype = C.c_int
+    L.orc_gas_force_on_bh.argtypes = [_pd, _i64, _i64, _i64] + [_d] * 11 + [_pd]
+    L.orc_accrete_sums.argtypes = [_pd, _i64, _i64, _i64] + [_d] * 13 + [_pd]
+    L.orc_gas_diagnostics.argtypes = [_pd, _i64, _i64, _i64] + [_d] * 6 + [C.c_int] + \
+        [_pd] * 5 + [_d, _pd]
 
 
 def _p(a):
@@ -311,3 +315,50 @@ def coupled_step(U, dx, dy, dz, dt, gamma, bc="outflow", rho_floor=0.0, P_floor=
                                 int(nsink), _p(r), _p(t), int(bool(torque_free)), rho_sink_min)
     if rc:
         raise ValueError(f"oracle coupled_step failed ({rc})")
+
+
+def gas_force_on_bh(U, dx, dy, dz, pos, mass, eps, x0, y0, z0):
+    """Force of the gas on one body (src/gravity_bh.jl:91-113), sequential sum."""
+    nx, ny, nz = _dims(U)
+    F = np.zeros(3)
+    lib().orc_gas_force_on_bh(_p(U), nx, ny, nz, dx, dy, dz, float(pos[0]), float(pos[1]),
+                              float(pos[2]), float(mass), float(eps), x0, y0, z0, _p(F))
+    return F
+
+
+def accrete_sums(U, dx, dy, dz, x0, y0, z0, pos, rs, ts, dt, rho_sink_min=0.0):
+    """(dm, dPx, dPy, dPz) of accrete! for one body (src/sinks.jl:86-103)."""
+    nx, ny, nz = _dims(U)
+    out = np.zeros(4)
+    lib().orc_accrete_sums(_p(U), nx, ny, nz, dx, dy, dz, x0, y0, z0, float(pos[0]),
+                           float(pos[1]), float(pos[2]), float(rs), float(ts), float(dt),
+                           float(rho_sink_min), _p(out))
+    return out
+
+
+def accrete(mass, vel, sums):
+    """The body update of accrete! (src/sinks.jl:104-108): (mass, vel) after
+    accreting `sums` = (dm, dPx, dPy, dPz)."""
+    dm = sums[0]
+    vel = [float(v) for v in vel]
+    if dm > 0.0:
+        M_new = mass + dm
+        vel = [(mass * vel[c] + sums[1 + c]) / M_new for c in range(3)]
+        mass = M_new
+    return mass, vel
+
+
+def gas_diagnostics(U, dx, dy, dz, x0, y0, z0, pos=(), mass=(), eps=(), gamma=5.0 / 3.0):
+    """(E, KE, Px, Py, Pz, Lx, Ly, Lz, M_bound) of src/diagnostics.jl:21-175,
+    sequential sums; pos holds 3 values per body."""
+    nx, ny, nz = _dims(U)
+    n = len(mass)
+    pos = np.asarray(pos, dtype=np.float64).reshape(n, 3) if n else np.zeros((1, 3))
+    px, py, pz = (np.ascontiguousarray(pos[:, c]) for c in range(3))
+    m = np.ascontiguousarray(mass, dtype=np.float64) if n else np.zeros(1)
+    e = np.ascontiguousarray(eps, dtype=np.float64) if n else np.zeros(1)
+    out = np.zeros(9)
+    lib().orc_gas_diagnostics(_p(U), nx, ny, nz, dx, dy, dz, x0, y0, z0, n, _p(px), _p(py),
+                              _p(pz), _p(m), _p(e), float(gamma), _p(out))
+    return out
+

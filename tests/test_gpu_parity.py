@@ -596,3 +596,114 @@ def test_step_with_self_gravity(form):
     for dt in (2e-3, 1.5e-3):
         O.euler3d_step(W, *d, dt, 1.4, "outflow", 1e-4, 1e-4)
     assert np.abs(W - V).max() > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# What the bodies and the diagnostics read from the gas.  The per-cell terms
+# are the oracle's bit for bit; the sums run in another order, so each sum is
+# held to 1e-12 of the sum of its terms' magnitudes (above the worst-case
+# rounding of either order at these sizes).  A cell counted differently in
+# accrete! or bound_gas_mass would show far above that.
+
+def _three_bodies():
+    B = bsn.BlackHole
+    return [B(pos=[0.13, -0.07, 0.05], vel=[0.1, 0.2, -0.1], mass=1.7, eps=0.03, c_code=1e6,
+              r_floor=0.33),
+            B(pos=[-0.93, 0.71, -0.41], vel=[0.0, -0.3, 0.2], mass=0.6, eps=0.02, c_code=1e6,
+              r_floor=0.25),                        # sink sphere clipped by the corner
+            B(pos=[2.4, -0.3, 0.2], vel=[-0.2, 0.0, 0.0], mass=40.0, eps=0.05, c_code=2.0,
+              r_floor=0.1)]                         # outside; r_sink 60 covers the grid
+
+
+GEOM = dict(n=(20, 14, 11), d=(0.1, 0.12, 0.09), o=(-1.0, -0.8, -0.5))
+
+
+def _force_scale(U, bh, d, o):
+    nx, ny, nz = GEOM["n"]
+    c = [o[a] + (np.arange(1, m + 1) - 0.5) * d[a] for a, m in enumerate((nx, ny, nz))]
+    Z, Y, X = np.meshgrid(c[2], c[1], c[0], indexing="ij")
+    dd = [X - bh.pos[0], Y - bh.pos[1], Z - bh.pos[2]]
+    r2 = dd[0] ** 2 + dd[1] ** 2 + dd[2] ** 2 + bh.eps ** 2
+    w = np.abs(U[NG:-NG, NG:-NG, NG:-NG, 0]) * bh.mass / (r2 * np.sqrt(r2)) * d[0] * d[1] * d[2]
+    return np.array([(w * np.abs(a)).sum() for a in dd])
+
+
+@pytest.mark.parametrize("wild", [False, True])
+def test_gas_force_on_bh_matches_oracle(wild):
+    print(f"gas force wild={wild}", flush=True)
+    (nx, ny, nz), d, o = GEOM["n"], GEOM["d"], GEOM["o"]
+    U = random_state(nx, ny, nz, seed=17, wild=wild)
+    g = bsn.Grid(nx, ny, nz).set_state(U)
+    bhs = _three_bodies()
+    F = bsn.gas_forces_on_bhs(g, nx, ny, nz, *d, bhs, *o)
+    for b, bh in enumerate(bhs):
+        ref = O.gas_force_on_bh(U, *d, bh.pos, bh.mass, bh.eps, *o)
+        assert np.all(np.abs(F[b] - ref) <= 1e-12 * _force_scale(U, bh, d, o)), (b, F[b], ref)
+    assert np.array_equal(bsn.gas_force_on_bh(g, nx, ny, nz, *d, bhs[1], *o), F[1])
+    # deterministic from call to call
+    assert np.array_equal(bsn.gas_forces_on_bhs(g, nx, ny, nz, *d, bhs, *o), F)
+
+
+@pytest.mark.parametrize("rho_sink_min", [0.0, 0.9])
+def test_accrete_matches_oracle(rho_sink_min):
+    import copy
+    print(f"accrete rho_sink_min={rho_sink_min}", flush=True)
+    (nx, ny, nz), d, o = GEOM["n"], GEOM["d"], GEOM["o"]
+    U = random_state(nx, ny, nz, seed=29, wild=False)
+    g = bsn.Grid(nx, ny, nz).set_state(U)
+    bhs = _three_bodies()
+    ref_bhs = copy.deepcopy(bhs)
+    dt, f_sink = 0.013, 1.5
+    dm = bsn.accrete_all_(bhs, g, nx, ny, nz, *d, *o, dt, f_sink=f_sink, rho_sink_min=rho_sink_min)
+    rho = U[NG:-NG, NG:-NG, NG:-NG, 0]
+    for b, (bh, rb) in enumerate(zip(bhs, ref_bhs)):
+        rs, ts = bsn.r_sink(rb), bsn.t_sink(rb, f_sink)
+        s = O.accrete_sums(U, *d, *o, rb.pos, rs, ts, dt, rho_sink_min)
+        mass, vel = O.accrete(rb.mass, rb.vel, s)
+        assert s[0] > 0
+        assert abs(dm[b] - s[0]) <= 1e-12 * s[0]
+        assert abs(bh.mass - mass) <= 1e-12 * s[0] + 1e-15 * mass
+        # v' = (M v + dP) / M': an error e in dP moves v' by e / M'
+        pscale = np.abs(U[NG:-NG, NG:-NG, NG:-NG, 1:4]).sum((0, 1, 2)) * (d[0] * d[1] * d[2] / ts * dt)
+        tol = 1e-12 * pscale / mass + 1e-14 * np.abs(vel) + 1e-300
+        assert np.all(np.abs(np.array(bh.vel) - np.array(vel)) <= tol), (b, bh.vel, vel)
+    # the third body's sphere covers the grid: it takes every cell above the threshold
+    rb = ref_bhs[2]
+    expect = (rho * (rho > rho_sink_min)).sum() * (d[0] * d[1] * d[2] / bsn.t_sink(rb, f_sink) * dt)
+    assert abs(dm[2] - expect) <= 1e-12 * expect
+
+
+@pytest.mark.parametrize("wild", [False, True])
+def test_gas_diagnostics_match_oracle(wild):
+    print(f"diagnostics wild={wild}", flush=True)
+    (nx, ny, nz), d, o = GEOM["n"], GEOM["d"], GEOM["o"]
+    gamma = 5.0 / 3.0
+    U = random_state(nx, ny, nz, seed=23, wild=wild, g=gamma)
+    g = bsn.Grid(nx, ny, nz).set_state(U)
+    bhs = _three_bodies()[:2]
+    bhs[0].mass = 2.5
+    D = bsn.diagnostics.gas_diagnostics(g, nx, ny, nz, *d, *o, bhs, gamma)
+    pos = [c for b in bhs for c in b.pos]
+    ref = O.gas_diagnostics(U, *d, *o, pos, [b.mass for b in bhs], [b.eps for b in bhs], gamma)
+    A = U[NG:-NG, NG:-NG, NG:-NG]
+    dV = d[0] * d[1] * d[2]
+    c = [o[a] + (np.arange(1, m + 1) - 0.5) * d[a] for a, m in enumerate((nx, ny, nz))]
+    Z, Y, X = np.meshgrid(c[2], c[1], c[0], indexing="ij")
+    rs = np.maximum(A[..., 0], 1e-30)
+    m = [A[..., q] for q in (1, 2, 3)]
+    ke = 0.5 * (m[0] ** 2 + m[1] ** 2 + m[2] ** 2) / rs
+    lsc = (np.abs(Y * m[2]) + np.abs(Z * m[1]) + np.abs(Z * m[0]) + np.abs(X * m[2]) +
+           np.abs(X * m[1]) + np.abs(Y * m[0])).sum()
+    scales = [np.abs(A[..., 4]).sum(), ke.sum()] + [np.abs(x).sum() for x in m] + [lsc] * 3
+    got = [D["gas_energy_total"], D["gas_kinetic_total"], *D["gas_momentum_total"],
+           *D["gas_angular_momentum_total"]]
+    for q in range(8):
+        assert abs(got[q] - ref[q]) <= 1e-12 * scales[q] * dV, (q, got[q], ref[q])
+    # bound mass: a sum of positive terms over the cells the per-cell test picks
+    assert abs(D["bound_gas_mass"] - ref[8]) <= 1e-13 * ref[8]
+    total = rs.sum() * dV
+    assert 0.05 * total < ref[8] < 0.95 * total
+    # the host-array form gives the same bits
+    D2 = bsn.diagnostics.gas_diagnostics(U.copy(), nx, ny, nz, *d, *o, bhs, gamma)
+    assert D2["bound_gas_mass"] == D["bound_gas_mass"]
+    assert np.array_equal(D2["gas_angular_momentum_total"], D["gas_angular_momentum_total"])
