@@ -19,7 +19,8 @@ export HIPGrid, euler3d_step!, euler3d_rhs!, cfl_dt_3d, apply_floors_3d!,
        fill_ghost_3d_outflow!, fill_ghost_3d_periodic!, rk3_step!, HIPFMR,
        fmr3d_step!, cfl_dt_fmr3d, set_phi!, download, add_bh_gravity_source!,
        add_sink_sources!, set_point_masses!, set_sinks!, solve_poisson_isolated,
-       set_self_gravity!
+       set_self_gravity!, gas_force_on_bh, accrete!, gas_diagnostics, gas_energy_total,
+       gas_kinetic_total, gas_momentum_total, gas_angular_momentum_total, bound_gas_mass
 
 const libbsn = joinpath(@__DIR__, "..", "libbsn_hip.so")
 const NG = 3
@@ -291,5 +292,57 @@ function set_self_gravity!(g::HIPGrid, on::Bool = true)
                 Cint(on)))
     return g
 end
+
+# ---------------------------------------------------------------------------
+# What the bodies and the diagnostics read from the gas (src/gravity_bh.jl:91,
+# src/sinks.jl:73, src/diagnostics.jl): device reductions, sums only to the host.
+
+function gas_force_on_bh(U::HIPGrid, nx::Int, ny::Int, nz::Int, dx::Real, dy::Real, dz::Real,
+                         bh, x0::Real, y0::Real, z0::Real)
+    F = zeros(3)
+    check(ccall((:bsn_gas_force_on_bh, libbsn), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Float64, Cint, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}, Float64, Float64, Float64, Ptr{Float64}),
+                U.handle, dx, dy, dz, Cint(1), bh_pos([bh]), Float64[bh.mass], Float64[bh.eps],
+                x0, y0, z0, F))
+    return F
+end
+
+function accrete!(bh, U::HIPGrid, nx::Int, ny::Int, nz::Int, dx::Real, dy::Real, dz::Real,
+                  x0::Real, y0::Real, z0::Real, dt::Float64, r_sink, t_sink;
+                  f_sink::Float64 = 1.0, ρ_sink_min::Float64 = 0.0)
+    vel = Float64[bh.vel[1], bh.vel[2], bh.vel[3]]
+    mass = Float64[bh.mass]
+    check(ccall((:bsn_accrete, libbsn), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Float64, Cint, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Float64,
+                 Float64, Ptr{Float64}),
+                U.handle, dx, dy, dz, Cint(1), bh_pos([bh]), vel, mass, Float64[r_sink(bh)],
+                Float64[t_sink(bh, f_sink)], ρ_sink_min, x0, y0, z0, dt, C_NULL))
+    bh.vel[1:3] .= vel
+    bh.mass = mass[1]
+    return nothing
+end
+
+"""All five gas diagnostics of src/diagnostics.jl in one pass."""
+function gas_diagnostics(U::HIPGrid, nx::Int, ny::Int, nz::Int, dx::Real, dy::Real, dz::Real,
+                         x0::Real = 0.0, y0::Real = 0.0, z0::Real = 0.0, bhs = (),
+                         γ::Real = 5 / 3)
+    out = zeros(9)
+    check(ccall((:bsn_gas_diagnostics, libbsn), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Float64, Float64, Float64, Float64, Cint,
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Ptr{Float64}),
+                U.handle, dx, dy, dz, x0, y0, z0, Cint(length(bhs)), bh_pos(bhs),
+                Float64[bh.mass for bh in bhs], Float64[bh.eps for bh in bhs], γ, out))
+    return out
+end
+
+gas_energy_total(U::HIPGrid, nx, ny, nz, dx, dy, dz) = gas_diagnostics(U, nx, ny, nz, dx, dy, dz)[1]
+gas_kinetic_total(U::HIPGrid, nx, ny, nz, dx, dy, dz) = gas_diagnostics(U, nx, ny, nz, dx, dy, dz)[2]
+gas_momentum_total(U::HIPGrid, nx, ny, nz, dx, dy, dz) = gas_diagnostics(U, nx, ny, nz, dx, dy, dz)[3:5]
+gas_angular_momentum_total(U::HIPGrid, nx, ny, nz, dx, dy, dz, x0, y0, z0) =
+    gas_diagnostics(U, nx, ny, nz, dx, dy, dz, x0, y0, z0)[6:8]
+bound_gas_mass(U::HIPGrid, nx, ny, nz, dx, dy, dz, x0, y0, z0, bhs, γ) =
+    gas_diagnostics(U, nx, ny, nz, dx, dy, dz, x0, y0, z0, bhs, γ)[9]
 
 end # module
