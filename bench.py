@@ -14,7 +14,8 @@ FP64 form is run and timed the same way for the speed-up and the L1 error.
 
 Other workloads (--config): sod128, sedov128, batch48, fmr3, selfgrav128 (the
 coupled step with an FFT Poisson solve every RK3 stage), binary128 (the coupled
-step with two bodies: gas forces, sinks, point masses, self-gravity, accretion).
+step with two bodies: gas forces, sinks, point masses, self-gravity, accretion),
+io256 (snapshot and checkpoint files streamed from and to the device).
 
 Prints ONE JSON line.  Inputs are resident in HBM before the timed region.
 """
@@ -618,6 +619,83 @@ def bench_binary(bsn, args):
     return out
 
 
+def bench_io(bsn, args):
+    """Snapshot and checkpoint files of the n^3 turbulent box (src/io.jl:29-51,
+    src/checkpoint.jl:244-283) written from and read into the GPU-resident
+    grid through pinned slabs, against the reference's way (copy the whole
+    state to the host, then write), into a temporary directory (page cache:
+    no fsync).  value = snapshot bytes written per second from the device."""
+    import ctypes as C
+    import shutil
+    import tempfile
+    from binary_supernova_amd import _lib
+    from binary_supernova_amd import io as bio
+    n = args.n or 256
+    U, dims, d = bsn.problems.turbulent_box(n, seed=1234)
+    g = bsn.Grid(n, n, n).set_state(U)
+    reps = args.steps
+    tmp = tempfile.mkdtemp(prefix="bsn_io_", dir=os.environ.get("BSN_IO_DIR", "/tmp"))
+    snap_bytes = 5 * n ** 3 * 8
+    ckpt_bytes = 5 * (n + 6) ** 3 * 8
+    kw = dict(t=1.0, step=10, dt_last=1e-3, t_snap=1.1, t_bh2_sink_on=0.0, nx=n, ny=n, nz=n,
+              dx=d[0], dy=d[1], dz=d[2], gamma=5 / 3, x0=0.0, y0=0.0, z0=0.0, rho_floor=0.0,
+              P_floor=0.0)
+
+    def timed(fn):
+        fn()                                          # warm-up (allocations, page cache)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    def noop(user, part, k0, nk, data, count):
+        return 0
+    cb = _lib.SLAB_FN(noop)
+    try:
+        snap, ckpt = os.path.join(tmp, "snap.h5"), os.path.join(tmp, "ckpt.h5")
+        t_pipe = timed(lambda: _lib.call("bsn_grid_export_slabs", g.handle, _lib.LAYOUT_FIELDS, 0,
+                                         cb, None))
+        t_snap = timed(lambda: bio.write_snapshot(snap, g, n, n, n, *d, 1.0, 5 / 3))
+        t_read = timed(lambda: bio.read_snapshot_(g, snap))
+        t_ckpt = timed(lambda: bio.save_checkpoint_uniform(ckpt, g, [], **kw))
+        t_load = timed(lambda: bio.load_checkpoint_uniform(ckpt, grid=g))
+        host = np.empty(g.shape)
+
+        def ref_snap():
+            g.get_state(host)
+            bio.write_snapshot(snap, host, n, n, n, *d, 1.0, 5 / 3)
+        t_ref = timed(ref_snap)
+
+        def ref_ckpt():
+            g.get_state(host)
+            bio.save_checkpoint_uniform(ckpt, host, [], **kw)
+        t_refc = timed(ref_ckpt)
+        ok = np.array_equal(g.get_state(), U)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    gbs = lambda b, t: b / t / 1e9
+    return dict(
+        metric="snapshot GB/s written from the GPU-resident state (HDF5, page cache)",
+        value=gbs(snap_bytes, t_snap), unit="GB/s", ms_per_step=1e3 * t_snap, dtype="f64",
+        config=dict(workload=f"snapshot and uniform checkpoint of the turbulent box {n}^3 "
+                             f"(HDF5 files of src/io.jl and src/checkpoint.jl), median of {reps}",
+                    grid=[n, n, n]),
+        snapshot=dict(bytes=snap_bytes, write_ms=1e3 * t_snap, read_ms=1e3 * t_read,
+                      write_gbs=gbs(snap_bytes, t_snap), read_gbs=gbs(snap_bytes, t_read),
+                      reference_way_ms=1e3 * t_ref,
+                      reference_way="whole state to the host, then write (src/io.jl:36-40)"),
+        checkpoint=dict(bytes=ckpt_bytes, save_ms=1e3 * t_ckpt, load_ms=1e3 * t_load,
+                        save_gbs=gbs(ckpt_bytes, t_ckpt), load_gbs=gbs(ckpt_bytes, t_load),
+                        reference_way_ms=1e3 * t_refc),
+        slab_pipeline=dict(ms=1e3 * t_pipe, gbs=gbs(snap_bytes, t_pipe),
+                           note="pack + PCIe copy of the five fields into pinned slabs, "
+                                "with a consumer that does nothing"),
+        state_restored_bitwise=bool(ok),
+    )
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -625,7 +703,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="turb256",
                     choices=["turb256", "sedov128", "sod128", "batch48", "fmr3", "selfgrav128",
-                             "binary128"])
+                             "binary128", "io256"])
     ap.add_argument("--n", type=int, default=None, help="override the grid size")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lean", action="store_true", help="skip the hipGraph replay pass "
@@ -658,6 +736,9 @@ def main():
     elif args.config == "binary128":
         args.steps = args.steps or 20
         out = bench_binary(bsn, args)
+    elif args.config == "io256":
+        args.steps = args.steps or 3
+        out = bench_io(bsn, args)
     else:
         args.steps = args.steps or 100
         out = bench_sod(bsn, args)

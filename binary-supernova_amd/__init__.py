@@ -5,7 +5,8 @@ The compute runs in libbsn_hip.so (hand-written HIP for gfx950, C ABI in
 include/bsn.h); this package is the host-side mirror of the reference's
 Julia entry points (src/euler3d.jl, src/rk3.jl, src/fmr3d.jl,
 src/fmr_gravity.jl, src/gravity_bh.jl, src/sinks.jl, src/self_gravity.jl,
-src/diagnostics.jl) plus batching and the benchmark workloads.
+src/diagnostics.jl, src/io.jl, src/checkpoint.jl) plus batching and the benchmark
+workloads.
 """
 from ._lib import BsnError, EXPORTS, LIB_PATH, load
 from .euler3d import (NG, Context, Grid, apply_floors_3d_, bc_mask, cfl_dt_3d,
@@ -18,6 +19,7 @@ from .sources import (BlackHole, accrete_, accrete_all_, add_bh_gravity_source_,
                       add_sink_sources_, gas_force_on_bh, gas_forces_on_bhs, r_sink,
                       set_point_masses, set_sinks, t_sink)
 from . import diagnostics
+from . import io
 from . import problems
 
 __all__ = [
@@ -28,4 +30,5 @@ __all__ = [
     "Batch", "problems", "BlackHole", "add_bh_gravity_source_", "add_sink_sources_", "r_sink",
     "set_point_masses", "set_sinks", "t_sink", "set_self_gravity", "solve_poisson_isolated",
     "accrete_", "accrete_all_", "gas_force_on_bh", "gas_forces_on_bhs", "diagnostics",
+    "io",
 ]

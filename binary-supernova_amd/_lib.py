@@ -26,6 +26,7 @@ EXPORTS = [
     "bsn_conserved_totals", "bsn_add_bh_gravity_source", "bsn_add_sink_sources",
     "bsn_grid_set_point_masses", "bsn_grid_set_sinks", "bsn_solve_poisson_isolated",
     "bsn_grid_set_self_gravity", "bsn_gas_force_on_bh", "bsn_accrete", "bsn_gas_diagnostics",
+    "bsn_grid_export_slabs", "bsn_grid_import_slabs", "bsn_grid_dims",
     "bsn_euler3d_step_host", "bsn_cfl_dt_3d_host", "bsn_batch_create",
     "bsn_batch_destroy", "bsn_batch_autotune", "bsn_batch_set_shapes", "bsn_batch_set_mode",
     "bsn_batch_set_update_shape", "bsn_batch_step",
@@ -42,6 +43,13 @@ class BsnError(RuntimeError):
 _lib = None
 _d, _i, _i64, _p = C.c_double, C.c_int, C.c_int64, C.c_void_p
 _pd = C.POINTER(C.c_double)
+
+
+# int (*)(void* user, int part, int64_t k0, int64_t nk, double* data, int64_t count)
+SLAB_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int64,
+                      C.POINTER(C.c_double), C.c_int64)
+LAYOUT_FIELDS = 0
+LAYOUT_REF = 1
 
 
 def load():
@@ -85,6 +93,9 @@ def load():
         "bsn_gas_force_on_bh": ([_p, _d, _d, _d, _i, _p, _p, _p, _d, _d, _d, _p], _i),
         "bsn_accrete": ([_p, _d, _d, _d, _i, _p, _p, _p, _p, _p, _d, _d, _d, _d, _d, _p], _i),
         "bsn_gas_diagnostics": ([_p, _d, _d, _d, _d, _d, _d, _i, _p, _p, _p, _d, _p], _i),
+        "bsn_grid_export_slabs": ([_p, _i, _i64, SLAB_FN, _p], _i),
+        "bsn_grid_import_slabs": ([_p, _i, _i64, SLAB_FN, _p], _i),
+        "bsn_grid_dims": ([_p, C.POINTER(_i64)], _i),
         "bsn_euler3d_step_host": ([_p, _i64, _i64, _i64, _d, _d, _d, _d, _d, _i, _d, _d, _i], _i),
         "bsn_cfl_dt_3d_host": ([_p, _i64, _i64, _i64, _d, _d, _d, _d, _d, _pd], _i),
         "bsn_batch_create": ([_p, _i, C.POINTER(_p), C.POINTER(_p)], _i),

@@ -23,6 +23,7 @@
 #include "bsn_launch.hpp"
 #include "bsn_poisson.hpp"
 #include "bsn_bodies.hpp"
+#include "bsn_slabs.hpp"
 #include "../../include/bsn.h"
 
 using namespace bsn;
@@ -85,6 +86,8 @@ struct bsn_grid {
     double* d_rho_in = nullptr;    // density staging for bsn_solve_poisson_isolated
     bool self_grav = false;        // phi solved from each stage's density
     double* d_body = nullptr;      // body/diagnostic reductions: partial sums, then the sums
+    double* h_pin = nullptr;       // two pinned slab buffers for file streaming
+    size_t pin_bytes = 0;          // bytes of each
     Plan* plan = nullptr;          // single-grid plan
 };
 
@@ -572,6 +575,7 @@ extern "C" int bsn_grid_destroy(bsn_grid* g) {
     if (g->pois) { poisson_destroy(*g->pois); delete g->pois; }
     (void)hipFree(g->d_rho_in);
     (void)hipFree(g->d_body);
+    if (g->h_pin) (void)hipHostFree(g->h_pin);
     delete g;
     return BSN_OK;
 }
@@ -1131,6 +1135,132 @@ extern "C" int bsn_gas_diagnostics(bsn_grid* g, double dx, double dy, double dz,
     CHK(body_finish(g, e, s, BQ_NACC_DIAG));
     for (int q = 0; q < BQ_NACC_DIAG; ++q) out[q] = s[q] * Q.dV;   // Float64(sum(...)) * dV
     return BSN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Streaming the state to and from files in k-slabs (bsn_slabs.hip)
+
+extern "C" int bsn_grid_dims(bsn_grid* g, int64_t n[3]) {
+    if (!g || !n) return set_err(BSN_ERR_ARG, "NULL argument");
+    n[0] = g->h.nx; n[1] = g->h.ny; n[2] = g->h.nz;
+    return BSN_OK;
+}
+
+struct SlabPlan {
+    int parts;
+    int64_t K;          // k-planes per part
+    int64_t per_k;      // values per k-plane
+    int64_t sk;         // k-planes per slab
+    int64_t nslab;      // slabs per part
+};
+
+static int slab_plan(bsn_grid* g, int layout, int64_t slab_k, SlabPlan& S) {
+    const GridDev& h = g->h;
+    if (layout == BSN_LAYOUT_FIELDS) {
+        S.parts = 5; S.K = h.nz; S.per_k = (int64_t)h.nx * h.ny;
+    } else if (layout == BSN_LAYOUT_REF) {
+        S.parts = 1; S.K = h.Nz; S.per_k = (int64_t)(h.nx + 2 * BSN_NG) * h.Ny * 5;
+    } else {
+        return set_err(BSN_ERR_ARG, "unknown layout %d", layout);
+    }
+    const int64_t target = (32LL << 20) / 8;            // about 32 MB per slab
+    int64_t sk = slab_k > 0 ? slab_k : target / S.per_k;
+    if (sk < 1) sk = 1;
+    if (sk > S.K) sk = S.K;
+    while (sk > 1 && sk * S.per_k >= (1LL << 31)) sk /= 2;   // 32-bit element index
+    if (sk * S.per_k >= (1LL << 31)) return set_err(BSN_ERR_ARG, "k-plane too large to stream");
+    S.sk = sk;
+    S.nslab = (S.K + sk - 1) / sk;
+    const size_t bytes = (size_t)(sk * S.per_k) * 8;
+    CHK(ensure_stage(g, 2 * bytes));
+    if (g->pin_bytes < bytes) {
+        if (g->h_pin) HIPCHK(hipHostFree(g->h_pin));
+        g->h_pin = nullptr;
+        g->pin_bytes = 0;
+        HIPCHK(hipHostMalloc((void**)&g->h_pin, 2 * bytes, hipHostMallocDefault));
+        g->pin_bytes = bytes;
+    }
+    return BSN_OK;
+}
+
+namespace {
+struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() { for (auto x : e) if (x) (void)hipEventDestroy(x); }
+};
+}  // namespace
+
+extern "C" int bsn_grid_export_slabs(bsn_grid* g, int layout, int64_t slab_k, bsn_slab_fn sink,
+                                     void* user) {
+    if (!g || !sink) return set_err(BSN_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    SlabPlan S;
+    CHK(slab_plan(g, layout, slab_k, S));
+    hipStream_t st = g->ctx->stream;
+    const int64_t slab_vals = S.sk * S.per_k;
+    double* pin[2] = {g->h_pin, g->h_pin + slab_vals};
+    double* dev[2] = {g->d_stage, g->d_stage + slab_vals};
+    Events ev;
+    for (auto& x : ev.e) HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    const int64_t total = S.parts * S.nslab;
+    int prev_part = 0;
+    int64_t prev_k0 = 0, prev_nk = 0;
+    for (int64_t s = 0; s <= total; ++s) {
+        const int b = (int)(s & 1);
+        int part = 0;
+        int64_t k0 = 0, nk = 0;
+        if (s < total) {
+            part = (int)(s / S.nslab);
+            k0 = (s % S.nslab) * S.sk;
+            nk = std::min(S.sk, S.K - k0);
+            HIPCHK(launch_slab(st, g->ctx->cus, g->h, 1, layout, part, (int)k0, (int)nk, dev[b]));
+            HIPCHK(hipMemcpyAsync(pin[b], dev[b], (size_t)(nk * S.per_k) * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipEventRecord(ev.e[b], st));
+        }
+        if (s > 0) {
+            // the previous slab is on the host once its copy has finished;
+            // the GPU meanwhile packs and copies slab s
+            HIPCHK(hipEventSynchronize(ev.e[1 - b]));
+            if (sink(user, prev_part, prev_k0, prev_nk, pin[1 - b], prev_nk * S.per_k) != 0) {
+                (void)hipStreamSynchronize(st);
+                return set_err(BSN_ERR_IO, "slab consumer failed (part %d, k0 %lld)", prev_part,
+                               (long long)prev_k0);
+            }
+        }
+        prev_part = part; prev_k0 = k0; prev_nk = nk;
+    }
+    return sync_ctx(g->ctx);
+}
+
+extern "C" int bsn_grid_import_slabs(bsn_grid* g, int layout, int64_t slab_k, bsn_slab_fn source,
+                                     void* user) {
+    if (!g || !source) return set_err(BSN_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    SlabPlan S;
+    CHK(slab_plan(g, layout, slab_k, S));
+    hipStream_t st = g->ctx->stream;
+    const int64_t slab_vals = S.sk * S.per_k;
+    double* pin[2] = {g->h_pin, g->h_pin + slab_vals};
+    double* dev[2] = {g->d_stage, g->d_stage + slab_vals};
+    Events ev;
+    for (auto& x : ev.e) HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    const int64_t total = S.parts * S.nslab;
+    for (int64_t s = 0; s < total; ++s) {
+        const int b = (int)(s & 1);
+        const int part = (int)(s / S.nslab);
+        const int64_t k0 = (s % S.nslab) * S.sk;
+        const int64_t nk = std::min(S.sk, S.K - k0);
+        // buffer b is free once slab s - 2 has been copied up and unpacked
+        if (s >= 2) HIPCHK(hipEventSynchronize(ev.e[b]));
+        if (source(user, part, k0, nk, pin[b], nk * S.per_k) != 0) {
+            (void)hipStreamSynchronize(st);
+            return set_err(BSN_ERR_IO, "slab producer failed (part %d, k0 %lld)", part, (long long)k0);
+        }
+        HIPCHK(hipMemcpyAsync(dev[b], pin[b], (size_t)(nk * S.per_k) * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(launch_slab(st, g->ctx->cus, g->h, 0, layout, part, (int)k0, (int)nk, dev[b]));
+        HIPCHK(hipEventRecord(ev.e[b], st));
+    }
+    return sync_ctx(g->ctx);
 }
 
 // Host-array drop-ins with a per-thread cached context and grid.

@@ -1,0 +1,168 @@
+"""Snapshot and checkpoint files (HDF5, the reference's layout) streamed
+between a GPU-resident grid and disk through libbsn_io.so.
+
+Mirrors write_snapshot / read_snapshot (src/io.jl:29-84) and
+save_checkpoint_uniform / load_checkpoint_uniform (src/checkpoint.jl:244-318).
+A Grid argument streams its state through pinned slabs (the device packs the
+file layout); a numpy array in the reference layout is written from the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from collections import namedtuple
+
+import numpy as np
+
+from ._lib import BSN_OK, BsnError, load
+from .euler3d import NG, Grid, _ptr
+from .sources import BlackHole
+
+IO_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbsn_io.so")
+EXPORTS = [
+    "bsn_io_last_error", "bsn_write_snapshot", "bsn_write_snapshot_host", "bsn_snapshot_info",
+    "bsn_read_snapshot", "bsn_read_snapshot_host", "bsn_save_checkpoint_uniform",
+    "bsn_checkpoint_uniform_info", "bsn_load_checkpoint_uniform",
+]
+
+_io = None
+_d, _i, _i64, _p, _s = C.c_double, C.c_int, C.c_int64, C.c_void_p, C.c_char_p
+_pd, _pi64 = C.POINTER(C.c_double), C.POINTER(C.c_int64)
+
+GridParams = namedtuple("GridParams", "nx ny nz dx dy dz gamma x0 y0 z0 rho_floor P_floor")
+CheckpointMeta = namedtuple("CheckpointMeta", "t step dt_last t_snap t_bh2_sink_on format_version")
+
+
+def load_io():
+    """Load libbsn_io.so (raises if it has not been built)."""
+    global _io
+    if _io is not None:
+        return _io
+    load()                                   # libbsn_hip.so first: libbsn_io links it
+    if not os.path.exists(IO_LIB_PATH):
+        raise BsnError(f"{IO_LIB_PATH} is missing: build it with binary-supernova_amd/build.py")
+    L = C.CDLL(IO_LIB_PATH)
+    sig = {
+        "bsn_io_last_error": ([], C.c_char_p),
+        "bsn_write_snapshot": ([_p, _s, _d, _d, _d, _d, _d], _i),
+        "bsn_write_snapshot_host": ([_s, _p, _i64, _i64, _i64, _d, _d, _d, _d, _d], _i),
+        "bsn_snapshot_info": ([_s, _pi64, _pd], _i),
+        "bsn_read_snapshot": ([_p, _s], _i),
+        "bsn_read_snapshot_host": ([_s, _p], _i),
+        "bsn_save_checkpoint_uniform": ([_p, _p, _i64, _i64, _i64, _s, _i64, _pd, _pd, _i, _p, _p,
+                                         _p], _i),
+        "bsn_checkpoint_uniform_info": ([_s, _pi64, C.POINTER(_i)], _i),
+        "bsn_load_checkpoint_uniform": ([_p, _p, _s, _pi64, _pd, _pd, _i, C.POINTER(_i), _p, _p,
+                                         _p], _i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _io = L
+    return L
+
+
+def _call(name, *args):
+    rc = getattr(load_io(), name)(*args)
+    if rc != BSN_OK:
+        msg = load_io().bsn_io_last_error().decode(errors="replace")
+        raise BsnError(f"{name}: status {rc}: {msg}")
+
+
+def _path(p) -> bytes:
+    return os.fsencode(os.fspath(p))
+
+
+def _state(U, nx, ny, nz):
+    if isinstance(U, Grid):
+        if (U.nx, U.ny, U.nz) != (nx, ny, nz):
+            raise ValueError("grid dims do not match nx, ny, nz")
+        return U, None
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    if U.shape != (nz + 2 * NG, ny + 2 * NG, nx + 2 * NG, 5):
+        raise ValueError(f"U shape {U.shape} does not match nx, ny, nz with ghosts")
+    return None, U
+
+
+def write_snapshot(filename, U, nx, ny, nz, dx, dy, dz, t, gamma) -> None:
+    """write_snapshot (src/io.jl:29-51): /gas/{rho, rhovx, rhovy, rhovz, E}
+    of the active cells and the attributes t, nx, ny, nz, dx, dy, dz, gamma."""
+    g, A = _state(U, nx, ny, nz)
+    if g is not None:
+        _call("bsn_write_snapshot", g.handle, _path(filename), float(dx), float(dy), float(dz),
+              float(t), float(gamma))
+    else:
+        _call("bsn_write_snapshot_host", _path(filename), _ptr(A), nx, ny, nz, float(dx),
+              float(dy), float(dz), float(t), float(gamma))
+
+
+def snapshot_info(filename):
+    n = (C.c_int64 * 3)()
+    meta = (C.c_double * 5)()
+    _call("bsn_snapshot_info", _path(filename), n, meta)
+    return tuple(n), tuple(meta)
+
+
+def read_snapshot(filename):
+    """read_snapshot (src/io.jl:60-84): (U_active, nx, ny, nz, dx, dy, dz, t,
+    gamma); U_active is the Julia (5, nx, ny, nz) array, here of shape
+    (nz, ny, nx, 5)."""
+    (nx, ny, nz), (dx, dy, dz, t, gamma) = snapshot_info(filename)
+    U = np.empty((nz, ny, nx, 5))
+    _call("bsn_read_snapshot_host", _path(filename), _ptr(U))
+    return U, nx, ny, nz, dx, dy, dz, t, gamma
+
+
+def read_snapshot_(grid: Grid, filename) -> None:
+    """The active cells of `grid` from a snapshot, streamed onto the device
+    (its ghost cells are left as they are)."""
+    _call("bsn_read_snapshot", grid.handle, _path(filename))
+
+
+def save_checkpoint_uniform(path, U, bhs, *, t, step, dt_last, t_snap, t_bh2_sink_on, nx, ny,
+                            nz, dx, dy, dz, gamma, x0, y0, z0, rho_floor, P_floor):
+    """save_checkpoint_uniform (src/checkpoint.jl:244-283)."""
+    g, A = _state(U, nx, ny, nz)
+    meta = (C.c_double * 4)(t, dt_last, t_snap, t_bh2_sink_on)
+    params = (C.c_double * 9)(dx, dy, dz, gamma, x0, y0, z0, rho_floor, P_floor)
+    n = len(bhs)
+    pos = np.ascontiguousarray([[float(c) for c in b.pos] for b in bhs], dtype=np.float64).reshape(n, 3)
+    vel = np.ascontiguousarray([[float(c) for c in b.vel] for b in bhs], dtype=np.float64).reshape(n, 3)
+    props = np.ascontiguousarray([[b.mass, b.eps, b.c_code, b.r_floor] for b in bhs],
+                                 dtype=np.float64).reshape(n, 4)
+    _call("bsn_save_checkpoint_uniform", g.handle if g is not None else None,
+          _ptr(A) if A is not None else None, nx, ny, nz, _path(path), int(step), meta, params, n,
+          _ptr(pos), _ptr(vel), _ptr(props))
+    return path
+
+
+def checkpoint_uniform_info(path):
+    n = (C.c_int64 * 3)()
+    nb = C.c_int()
+    _call("bsn_checkpoint_uniform_info", _path(path), n, C.byref(nb))
+    return tuple(n), nb.value
+
+
+def load_checkpoint_uniform(path, grid: Grid | None = None):
+    """load_checkpoint_uniform (src/checkpoint.jl:285-318): (U, bhs, grid
+    params, meta).  With `grid` the state is streamed into it and U is the
+    grid; otherwise U is a host array in the reference layout."""
+    (nx, ny, nz), nb = checkpoint_uniform_info(path)
+    U = None
+    if grid is None:
+        U = np.empty((nz + 2 * NG, ny + 2 * NG, nx + 2 * NG, 5))
+    step = C.c_int64()
+    meta = (C.c_double * 5)()
+    params = (C.c_double * 9)()
+    got = C.c_int()
+    pos, vel, props = np.zeros((max(nb, 1), 3)), np.zeros((max(nb, 1), 3)), np.zeros((max(nb, 1), 4))
+    _call("bsn_load_checkpoint_uniform", grid.handle if grid is not None else None,
+          _ptr(U) if U is not None else None, _path(path), C.byref(step), meta, params, max(nb, 1),
+          C.byref(got), _ptr(pos), _ptr(vel), _ptr(props))
+    bhs = [BlackHole(pos=list(pos[i]), vel=list(vel[i]), mass=float(props[i, 0]),
+                     eps=float(props[i, 1]), c_code=float(props[i, 2]),
+                     r_floor=float(props[i, 3])) for i in range(got.value)]
+    gp = GridParams(nx, ny, nz, *params)
+    m = CheckpointMeta(meta[0], step.value, meta[1], meta[2], meta[3], int(meta[4]))
+    return (grid if grid is not None else U), bhs, gp, m

@@ -35,6 +35,7 @@ extern "C" {
 #define BSN_ERR_HIP 2        /* a HIP runtime call failed */
 #define BSN_ERR_NONFINITE 3  /* non-finite value detected where checked */
 #define BSN_ERR_STATE 4      /* call not valid in the object's state */
+#define BSN_ERR_IO 5         /* a file could not be read or written, or holds the wrong data */
 
 #define BSN_BC_OUTFLOW 0
 #define BSN_BC_PERIODIC 7
@@ -178,6 +179,31 @@ int bsn_accrete(bsn_grid* g, double dx, double dy, double dz, int n, const doubl
 int bsn_gas_diagnostics(bsn_grid* g, double dx, double dy, double dz, double x0, double y0,
                         double z0, int n, const double* pos, const double* mass,
                         const double* eps, double gamma, double out[9]);
+
+/* Streaming a grid's state to and from files ------------------------------
+ * The state crosses PCIe in slabs of k-planes through two pinned host
+ * buffers: for each slab the device packs the file layout, copies it to one
+ * buffer and the callback consumes it while the next slab is packed and copied
+ * into the other (export); or the callback fills one buffer while the previous
+ * slab is copied up and unpacked (import).  Layouts:
+ *   BSN_LAYOUT_FIELDS  five parts, part q = field q of the active cells as a
+ *                      Julia (nx, ny, nz) array: the /gas datasets of
+ *                      write_snapshot (src/io.jl:29-51); k runs over 0..nz-1;
+ *   BSN_LAYOUT_REF     one part, the reference state array with ghosts,
+ *                      (5, nx+6, ny+6, nz+6): /grid/U of save_checkpoint_uniform
+ *                      (src/checkpoint.jl:244-283); k runs over 0..nz+5.
+ * A slab holds nk k-planes starting at k0, count values in the layout's order
+ * (k slowest).  slab_k <= 0 picks about 32 MB per slab.  A callback returning
+ * nonzero stops the stream with BSN_ERR_IO.  data is valid only during the call. */
+#define BSN_LAYOUT_FIELDS 0
+#define BSN_LAYOUT_REF 1
+typedef int (*bsn_slab_fn)(void* user, int part, int64_t k0, int64_t nk, double* data,
+                           int64_t count);
+int bsn_grid_export_slabs(bsn_grid* g, int layout, int64_t slab_k, bsn_slab_fn sink, void* user);
+int bsn_grid_import_slabs(bsn_grid* g, int layout, int64_t slab_k, bsn_slab_fn source,
+                          void* user);
+/* The grid's active extents (nx, ny, nz). */
+int bsn_grid_dims(bsn_grid* g, int64_t n[3]);
 
 /* Host-array convenience drop-ins: the reference signature on a host Julia
  * Array U; a cached per-thread grid uploads, runs and downloads. */
