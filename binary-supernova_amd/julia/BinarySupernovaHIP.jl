@@ -20,9 +20,11 @@ export HIPGrid, euler3d_step!, euler3d_rhs!, cfl_dt_3d, apply_floors_3d!,
        fmr3d_step!, cfl_dt_fmr3d, set_phi!, download, add_bh_gravity_source!,
        add_sink_sources!, set_point_masses!, set_sinks!, solve_poisson_isolated,
        set_self_gravity!, gas_force_on_bh, accrete!, gas_diagnostics, gas_energy_total,
-       gas_kinetic_total, gas_momentum_total, gas_angular_momentum_total, bound_gas_mass
+       gas_kinetic_total, gas_momentum_total, gas_angular_momentum_total, bound_gas_mass,
+       write_snapshot, read_snapshot!, save_checkpoint_uniform, load_checkpoint_uniform!
 
 const libbsn = joinpath(@__DIR__, "..", "libbsn_hip.so")
+const libbsn_io = joinpath(@__DIR__, "..", "libbsn_io.so")
 const NG = 3
 const BSN_HOST = Cint(0)
 
@@ -344,5 +346,67 @@ gas_angular_momentum_total(U::HIPGrid, nx, ny, nz, dx, dy, dz, x0, y0, z0) =
     gas_diagnostics(U, nx, ny, nz, dx, dy, dz, x0, y0, z0)[6:8]
 bound_gas_mass(U::HIPGrid, nx, ny, nz, dx, dy, dz, x0, y0, z0, bhs, γ) =
     gas_diagnostics(U, nx, ny, nz, dx, dy, dz, x0, y0, z0, bhs, γ)[9]
+
+# ---------------------------------------------------------------------------
+# Snapshot and checkpoint files (src/io.jl:29-84, src/checkpoint.jl:244-318),
+# streamed between the device state and HDF5 through pinned slabs.
+
+function checkio(rc::Cint)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:bsn_io_last_error, libbsn_io), Cstring, ()))
+    error("libbsn_io: status $rc: $msg")
+end
+
+function write_snapshot(filename::String, U::HIPGrid, nx::Int, ny::Int, nz::Int,
+                        dx::Real, dy::Real, dz::Real, t::Real, γ::Real)
+    checkio(ccall((:bsn_write_snapshot, libbsn_io), Cint,
+                  (Ptr{Cvoid}, Cstring, Float64, Float64, Float64, Float64, Float64),
+                  U.handle, filename, dx, dy, dz, t, γ))
+    return nothing
+end
+
+"""The active cells of U from a snapshot (its ghost cells are left as they are)."""
+function read_snapshot!(U::HIPGrid, filename::String)
+    checkio(ccall((:bsn_read_snapshot, libbsn_io), Cint, (Ptr{Cvoid}, Cstring),
+                  U.handle, filename))
+    return U
+end
+
+function save_checkpoint_uniform(path::AbstractString, U::HIPGrid, bhs;
+                                 t::Real, step::Integer, dt_last::Real, t_snap::Real,
+                                 t_bh2_sink_on::Real, nx::Integer, ny::Integer, nz::Integer,
+                                 dx::Real, dy::Real, dz::Real, γ::Real, x0::Real, y0::Real,
+                                 z0::Real, ρ_floor::Real, P_floor::Real)
+    meta = Float64[t, dt_last, t_snap, t_bh2_sink_on]
+    params = Float64[dx, dy, dz, γ, x0, y0, z0, ρ_floor, P_floor]
+    props = Float64[p for bh in bhs for p in (bh.mass, bh.eps, bh.c_code, bh.r_floor)]
+    checkio(ccall((:bsn_save_checkpoint_uniform, libbsn_io), Cint,
+                  (Ptr{Cvoid}, Ptr{Float64}, Int64, Int64, Int64, Cstring, Int64, Ptr{Float64},
+                   Ptr{Float64}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                  U.handle, C_NULL, nx, ny, nz, path, step, meta, params, Cint(length(bhs)),
+                  bh_pos(bhs), bh_vel(bhs), props))
+    return path
+end
+
+"""The state of a uniform checkpoint into U; returns (bodies as (pos, vel, mass,
+eps, c_code, r_floor) tuples, grid params, meta)."""
+function load_checkpoint_uniform!(U::HIPGrid, path::AbstractString; max_bodies::Int = 8)
+    step = Ref{Int64}(0)
+    meta = zeros(5); params = zeros(9); nb = Ref{Cint}(0)
+    pos = zeros(3 * max_bodies); vel = zeros(3 * max_bodies); props = zeros(4 * max_bodies)
+    checkio(ccall((:bsn_load_checkpoint_uniform, libbsn_io), Cint,
+                  (Ptr{Cvoid}, Ptr{Float64}, Cstring, Ref{Int64}, Ptr{Float64}, Ptr{Float64}, Cint,
+                   Ref{Cint}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                  U.handle, C_NULL, path, step, meta, params, Cint(max_bodies), nb, pos, vel,
+                  props))
+    bodies = [(pos[3i-2:3i], vel[3i-2:3i], props[4i-3], props[4i-2], props[4i-1], props[4i])
+              for i in 1:nb[]]
+    grid = (; nx = U.nx, ny = U.ny, nz = U.nz, dx = params[1], dy = params[2], dz = params[3],
+            γ = params[4], x0 = params[5], y0 = params[6], z0 = params[7],
+            ρ_floor = params[8], P_floor = params[9])
+    m = (; t = meta[1], step = step[], dt_last = meta[2], t_snap = meta[3],
+         t_bh2_sink_on = meta[4], format_version = Int(meta[5]))
+    return bodies, grid, m
+end
 
 end # module

@@ -37,7 +37,7 @@ def test_bc_masks():
 
 
 def header_arg_counts():
-    text = (ROOT / "include" / "bsn.h").read_text()
+    text = (ROOT / "include" / "bsn.h").read_text() + (ROOT / "include" / "bsn_io.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     out = {}
     for name, args in re.findall(r"\b(bsn_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
@@ -51,9 +51,9 @@ def test_julia_ccalls_match_the_header():
     # declared number of arguments (the binding cannot be run here)
     src = (ROOT / "binary-supernova_amd" / "julia" / "BinarySupernovaHIP.jl").read_text()
     decl = header_arg_counts()
-    calls = re.findall(r"ccall\(\(:(bsn_[a-z0-9_]+),\s*libbsn\),\s*\w+,\s*\(([^()]*(?:\([^()]*\)[^()]*)*)\)",
+    calls = re.findall(r"ccall\(\(:(bsn_[a-z0-9_]+),\s*libbsn(?:_io)?\),\s*\w+,\s*\(([^()]*(?:\([^()]*\)[^()]*)*)\)",
                        src, flags=re.S)
-    assert len(calls) >= 20
+    assert len(calls) == len(re.findall(r"ccall\(", src)) >= 30
     for name, types in calls:
         assert name in decl, name
         t = types.strip().rstrip(",")
