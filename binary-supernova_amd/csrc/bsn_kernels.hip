@@ -139,7 +139,17 @@ __global__ void k_ghost_combine(const GridDev* __restrict__ grids, const int64_t
     }
 }
 
+// a / b: IEEE division for the FP64 form; for the split form (FAST) the
+// product with a reciprocal good to a few ulp, which its FP64 state needs for
+// range and accuracy, not for bit parity.
+template <bool FAST>
+__device__ __forceinline__ double ddiv(double a, double b) {
+    if (FAST) return a * sp::rcp64(b);
+    return a / b;
+}
+
 // src/gpu_kernels.jl:26-46 on the five values of one cell.
+template <bool FAST = false>
 __device__ __forceinline__ bool apply_floor(double (&u)[5], double rho_floor, double P_floor,
                                             double g) {
     const double r = u[0];
@@ -148,7 +158,7 @@ __device__ __forceinline__ bool apply_floor(double (&u)[5], double rho_floor, do
         u[4] = P_floor / (g - 1.0);
         return true;
     }
-    const double KE = (0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3])) / r;
+    const double KE = ddiv<FAST>(0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]), r);
     const double P = (g - 1.0) * (u[4] - KE);
     if (P < P_floor) { u[4] = P_floor / (g - 1.0) + KE; return true; }
     return false;
@@ -159,7 +169,7 @@ __device__ __forceinline__ bool apply_floor(double (&u)[5], double rho_floor, do
 // reads the active cell its index maps to (the value the reference's ghost
 // fill would have copied there, unfloored: the fill precedes the floors,
 // euler3d.jl:189-196); with lmode bit 1 an active cell is floored on load.
-template <int AX, bool FLOOR = true>
+template <int AX, bool FLOOR = true, bool FAST = false>
 __device__ __forceinline__ void load_cell(const GridDev& G, int i, int j, int k, double (&u)[5]) {
     int si = i, sj = j, sk = k;
     if (G.lmode & 1) {
@@ -171,7 +181,7 @@ __device__ __forceinline__ void load_cell(const GridDev& G, int i, int j, int k,
 #pragma unroll
     for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + o];
     if (FLOOR && (G.lmode & 2) && si == i && sj == j && sk == k)
-        apply_floor(u, G.rho_floor, G.P_floor, G.gamma);
+        apply_floor<FAST>(u, G.rho_floor, G.P_floor, G.gamma);
 }
 
 // Floors on active cells (apply_floors_3d!, euler3d.jl:73-80); with copy_un
@@ -222,6 +232,9 @@ template <> struct Prec<double> {
                                 double F[5]) {
         ex::interface_flux<DIR>(c, p, G.gamma, G.gfac, F);
     }
+    __device__ static double pressure(const double (&u)[5], double g) {
+        return ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], g);
+    }
 };
 template <> struct Prec<float> {
     template <int DIR>
@@ -229,12 +242,16 @@ template <> struct Prec<float> {
                                 float F[5]) {
         sp::interface_flux<DIR>(c, p, G.gf32[0], G.gf32[1], G.gf32[2], F);
     }
+    __device__ static double pressure(const double (&u)[5], double g) {
+        return sp::cell_pressure(u[0], u[1], u[2], u[3], u[4], g);
+    }
 };
 
 template <int DIR, typename T, int W, int H>
 __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grids,
                                                  const Tile* __restrict__ tiles, int ntiles) {
     constexpr int NT = W * H;
+    constexpr bool FAST = sizeof(T) == 4;        // split form
     constexpr int SA = DIR == 0 ? H : H + 5;     // LDS rows
     constexpr int SB = DIR == 0 ? W + 5 : W;     // LDS row length
     __shared__ T lds[5][SA][SB];
@@ -268,10 +285,10 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
 #pragma unroll
                 for (int m = 0; m < 6; ++m) {
                     double u[5];
-                    if (DIR == 0)      load_cell<0>(G, f + m, NG + ci, NG + cj, u);
-                    else if (DIR == 1) load_cell<1>(G, NG + ci, f + m, NG + cj, u);
-                    else               load_cell<2>(G, NG + ci, NG + cj, f + m, u);
-                    const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
+                    if (DIR == 0)      load_cell<0, true, FAST>(G, f + m, NG + ci, NG + cj, u);
+                    else if (DIR == 1) load_cell<1, true, FAST>(G, NG + ci, f + m, NG + cj, u);
+                    else               load_cell<2, true, FAST>(G, NG + ci, NG + cj, f + m, u);
+                    const double pr = Prec<T>::pressure(u, G.gamma);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) c[q][m] = (T)u[q];
                     p[m] = (T)pr;
@@ -297,8 +314,8 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
             }
             if (ok) {
                 double u[5];
-                load_cell<DIR>(G, pi, pj, pk, u);
-                const double pr = ex::cell_pressure(u[0], u[1], u[2], u[3], u[4], G.gamma);
+                load_cell<DIR, true, FAST>(G, pi, pj, pk, u);
+                const double pr = Prec<T>::pressure(u, G.gamma);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) lds[q][a][b] = (T)u[q];
                 lds[4][a][b] = (T)pr;
@@ -336,14 +353,15 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
 // cells along z and carries the z-face flux.  stage 0 stores dU instead
 // (euler3d_rhs!).
 
+template <bool FAST = false>
 __device__ __forceinline__ uint64_t cfl_speed_bits(const double (&u)[5], const GridDev& G) {
     // src/gpu_kernels.jl:180-189
     const double g = G.gamma;
     const double r = jl_max(u[0], 1e-30);
-    const double vx = u[1] / r, vy = u[2] / r, vz = u[3] / r;
+    const double vx = ddiv<FAST>(u[1], r), vy = ddiv<FAST>(u[2], r), vz = ddiv<FAST>(u[3], r);
     const double KE = 0.5 * (vx * vx + vy * vy + vz * vz);
-    const double P = jl_max(((g - 1.0) * (u[4] / r - KE)) * r, 0.0);
-    const double cs = sqrt((g * P) / r);
+    const double P = jl_max(((g - 1.0) * (ddiv<FAST>(u[4], r) - KE)) * r, 0.0);
+    const double cs = sqrt(ddiv<FAST>(g * P, r));
     const double s = jl_max(jl_max((fabs(vx) + cs) * G.idx, (fabs(vy) + cs) * G.idy),
                             (fabs(vz) + cs) * G.idz);
     return (uint64_t)__double_as_longlong(s);
@@ -434,6 +452,7 @@ __device__ __forceinline__ void sink_add(const SrcDev& S, int i, int j, int k,
 // Phi source, stage combination and floors of active cell (i, j, k) with the
 // flux divergence d (src/gpu_kernels.jl:291-319, src/euler3d.jl:244-257);
 // the result goes to Uout.  Returns the CFL speed bits of the new state.
+template <bool FAST = false>
 __device__ __forceinline__ void cell_state(const GridDev& G, int64_t c, int stage, double (&u)[5],
                                            double (&un)[5]) {
     const bool need_u = stage > 1 || G.phi != nullptr || G.src != nullptr;
@@ -442,9 +461,10 @@ __device__ __forceinline__ void cell_state(const GridDev& G, int64_t c, int stag
         un[q] = G.Un[q * G.fs + c];
         u[q] = need_u ? G.U[q * G.fs + c] : 0.0;
     }
-    if (need_u && (G.lmode & 2)) apply_floor(u, G.rho_floor, G.P_floor, G.gamma);
+    if (need_u && (G.lmode & 2)) apply_floor<FAST>(u, G.rho_floor, G.P_floor, G.gamma);
 }
 
+template <bool FAST = false>
 __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, int j, int k, int stage,
                                                      double (&d)[5], const double (&u)[5],
                                                      const double (&un)[5], int do_cfl, double dt) {
@@ -482,17 +502,18 @@ __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, in
         else if (stage == 2) v[q] = 0.75 * un[q] + 0.25 * u[q] + (0.25 * dt) * d[q];
         else                 v[q] = (1.0 / 3.0) * un[q] + (2.0 / 3.0) * u[q] + ((2.0 / 3.0) * dt) * d[q];
     }
-    apply_floor(v, G.rho_floor, G.P_floor, G.gamma);
+    apply_floor<FAST>(v, G.rho_floor, G.P_floor, G.gamma);
 #pragma unroll
     for (int q = 0; q < 5; ++q) G.Uout[q * G.fs + c] = v[q];
-    return do_cfl ? cfl_speed_bits(v, G) : 0;
+    return do_cfl ? cfl_speed_bits<FAST>(v, G) : 0;
 }
 
+template <bool FAST = false>
 __device__ __forceinline__ uint64_t finish_cell(const GridDev& G, int i, int j, int k, int stage,
                                                 double (&d)[5], int do_cfl) {
     double u[5], un[5];
-    cell_state(G, uidx(G, NG + i, NG + j, NG + k), stage, u, un);
-    return finish_cell_with(G, i, j, k, stage, d, u, un, do_cfl, G.scal[SCAL_DT]);
+    cell_state<FAST>(G, uidx(G, NG + i, NG + j, NG + k), stage, u, un);
+    return finish_cell_with<FAST>(G, i, j, k, stage, d, u, un, do_cfl, G.scal[SCAL_DT]);
 }
 
 template <typename T, int KZ>
@@ -538,7 +559,7 @@ __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grid
 #pragma unroll
                     for (int q = 0; q < 5; ++q) dU_out[q * G.fs + c] = d[q];
                 } else {
-                    const uint64_t b = finish_cell(G, i, j, k, stage, d, do_cfl);
+                    const uint64_t b = finish_cell<sizeof(T) == 4>(G, i, j, k, stage, d, do_cfl);
                     smax = b > smax ? b : smax;
                 }
 #pragma unroll
@@ -567,6 +588,7 @@ template <typename T, int PF, int WB, int RING>
 __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__ grids,
                                                      const Tile* __restrict__ tiles, int ntiles,
                                                      int stage, int do_cfl) {
+    constexpr bool FAST = sizeof(T) == 4;        // split form
     __shared__ double ring[RING ? 4 : 1][5][RING ? 256 : 1];
     const int rt = RING ? (int)threadIdx.x : 0;
     int t0, t1;
@@ -592,7 +614,7 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
             int npz = 0;
             auto fetch = [&](int pz) {
                 npz = pz;
-                load_cell<2, !RING>(G, NG + i, NG + j, pz, nxt);
+                load_cell<2, !RING, FAST>(G, NG + i, NG + j, pz, nxt);
             };
             auto place = [&](int slot) {
                 if (RING) {
@@ -601,9 +623,9 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
 #pragma unroll
                     for (int q = 0; q < 5; ++q) ring[npz & 3][q][rt] = nxt[q];
                     if ((G.lmode & 2) && npz >= NG && npz < NG + G.nz)
-                        apply_floor(nxt, G.rho_floor, G.P_floor, G.gamma);
+                        apply_floor<FAST>(nxt, G.rho_floor, G.P_floor, G.gamma);
                 }
-                const double pr = ex::cell_pressure(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], G.gamma);
+                const double pr = Prec<T>::pressure(nxt, G.gamma);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) c[q][slot] = (T)nxt[q];
                 p[slot] = (T)pr;
@@ -632,15 +654,15 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
                         u[q] = r;
                         un[q] = un_in_ring ? r : G.Un[q * G.fs + cc];
                     }
-                    if (need_u && (G.lmode & 2)) apply_floor(u, G.rho_floor, G.P_floor, G.gamma);
+                    if (need_u && (G.lmode & 2)) apply_floor<FAST>(u, G.rho_floor, G.P_floor, G.gamma);
                 } else {
-                    cell_state(G, cc, stage, u, un);
+                    cell_state<FAST>(G, cc, stage, u, un);
                 }
             };
 #pragma unroll
             for (int m = 0; m < 5; ++m) { fetch(k0 + m); place(m); }
             fetch(k0 + 5);
-            double fz0[5];
+            T fz0[5];                                   // the previous interface's z flux
             for (int f = k0; f <= kend; ++f) {
                 place(5);                               // cell f+5
                 if (PF >= 1 && f < kend) fetch(f + 6);  // next iteration's cell, in flight during the flux
@@ -656,12 +678,12 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
                     for (int q = 0; q < 5; ++q)
                         d[q] = (-((double)fx[1][q] - (double)fx[0][q])) * G.idx -
                                ((double)fy[1][q] - (double)fy[0][q]) * G.idy -
-                               ((double)F[q] - fz0[q]) * G.idz;
-                    const uint64_t b = finish_cell_with(G, i, j, k, stage, d, u, un, do_cfl, dt);
+                               ((double)F[q] - (double)fz0[q]) * G.idz;
+                    const uint64_t b = finish_cell_with<FAST>(G, i, j, k, stage, d, u, un, do_cfl, dt);
                     smax = b > smax ? b : smax;
                 }
 #pragma unroll
-                for (int q = 0; q < 5; ++q) fz0[q] = (double)F[q];
+                for (int q = 0; q < 5; ++q) fz0[q] = F[q];
 #pragma unroll
                 for (int m = 0; m < 5; ++m) {
 #pragma unroll

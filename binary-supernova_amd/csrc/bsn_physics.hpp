@@ -242,6 +242,23 @@ namespace sp {
 #pragma clang fp contract(fast)
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// 1/x in FP64 from the hardware reciprocal and one Newton step (relative
+// error a few ulp, not correctly rounded): the split form's cell pressure
+// needs FP64 range and about 1e-15 relative accuracy, not IEEE division.
+__device__ __forceinline__ double rcp64(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
+// Cell pressure for the FP32 stencils (src/euler3d.jl:86-90), formed in FP64
+// so that E - rho v^2 / 2 does not cancel in FP32.
+__device__ __forceinline__ double cell_pressure(double r, double mx, double my, double mz, double E,
+                                                double g) {
+    r = jl_max(r, 1e-30);
+    const double KE = 0.5 * (mx * mx + my * my + mz * mz) * rcp64(r);
+    return jl_max((g - 1.0) * (E - KE), 0.0);
+}
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fmaxj(float x, float y) { return x > y ? x : y; }
 __device__ __forceinline__ float fminj(float x, float y) { return x < y ? x : y; }
@@ -357,26 +374,22 @@ __device__ __forceinline__ void hllc_normal(float rL, float mnL, float mt1L, flo
     // Star-state jumps written through del = (S* - vn) / (S - S*), so that
     // chi = rho (1 + del) and every jump U* - U vanishes exactly when S* = vn
     // (identical states at rest give zero mass and energy flux in FP32 too);
-    // algebraically the reference's expressions (src/hllc.jl:140-165).
-    if (S_star >= 0.0f) {
-        float del = (S_star - vnL) * rcp(SL - S_star);
-        float chi = rL + rL * del;
-        float dE = fabsf(dL) > 1e-37f ? EL * del + chi * (S_star - vnL) * (S_star + pL * rcp(dL)) : -EL;
-        F[0] = mnL + SL * (rL * del);
-        F[1] = FmnL + SL * (chi * S_star - mnL);
-        F[2] = mt1L * vnL + SL * (mt1L * del);
-        F[3] = mt2L * vnL + SL * (mt2L * del);
-        F[4] = FEL + SL * dE;
-    } else {
-        float del = (S_star - vnR) * rcp(SR - S_star);
-        float chi = rR + rR * del;
-        float dE = fabsf(dR) > 1e-37f ? ER * del + chi * (S_star - vnR) * (S_star + pR * rcp(dR)) : -ER;
-        F[0] = mnR + SR * (rR * del);
-        F[1] = FmnR + SR * (chi * S_star - mnR);
-        F[2] = mt1R * vnR + SR * (mt1R * del);
-        F[3] = mt2R * vnR + SR * (mt2R * del);
-        F[4] = FER + SR * dE;
-    }
+    // algebraically the reference's expressions (src/hllc.jl:140-165).  The
+    // side (left star state if S* >= 0, else right) is chosen per lane first,
+    // so a wave whose lanes differ evaluates the star formulas once.
+    const bool lft = S_star >= 0.0f;
+    const float r = lft ? rL : rR, mn = lft ? mnL : mnR, mt1 = lft ? mt1L : mt1R;
+    const float mt2 = lft ? mt2L : mt2R, E = lft ? EL : ER, p = lft ? pL : pR;
+    const float vn = lft ? vnL : vnR, S = lft ? SL : SR, d = lft ? dL : dR;
+    const float Fmn = lft ? FmnL : FmnR, FE = lft ? FEL : FER;
+    float del = (S_star - vn) * rcp(S - S_star);
+    float chi = r + r * del;
+    float dE = fabsf(d) > 1e-37f ? E * del + chi * (S_star - vn) * (S_star + p * rcp(d)) : -E;
+    F[0] = mn + S * (r * del);
+    F[1] = Fmn + S * (chi * S_star - mn);
+    F[2] = mt1 * vn + S * (mt1 * del);
+    F[3] = mt2 * vn + S * (mt2 * del);
+    F[4] = FE + S * dE;
 }
 
 template <int DIR>
