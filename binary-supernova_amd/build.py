@@ -22,8 +22,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsn_hip.so")
 IO_LIB = os.path.join(HERE, "libbsn_io.so")
 HDF5_ROOT = os.environ.get("BSN_HDF5_ROOT", "/opt/conda")
-SOURCES = ["bsn_kernels.hip", "bsn_poisson.hip", "bsn_bodies.hip", "bsn_slabs.hip",
-           "bsn_api.cpp"]
+SOURCES = ["bsn_kernels.hip", "bsn_poisson.hip", "bsn_fft.hip", "bsn_bodies.hip",
+           "bsn_slabs.hip", "bsn_api.cpp"]
 ARCH = os.environ.get("BSN_OFFLOAD_ARCH", "gfx950")
 
 

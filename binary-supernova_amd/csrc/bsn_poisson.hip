@@ -4,11 +4,15 @@
 // Hockney-Eastwood zero padding: the active-cell density is placed in the
 // first octant of a (2nx, 2ny, 2nz) array, convolved by FFT with the softened
 // kernel K(r) = -1/sqrt(r^2 + eps^2), eps^2 = (dx dy dz)^(2/3), and the first
-// octant of dV * IFFT(FFT(K) FFT(rho_pad)) is the potential.  The transforms
-// are real-to-complex hipFFT plans of the padded array (x fastest); the kernel
-// spectrum is computed once per grid and cell size.
+// octant of dV * IFFT(FFT(K) FFT(rho_pad)) is the potential.  The kernel
+// spectrum is computed once per grid and cell size with real-to-complex hipFFT
+// plans of the padded array (x fastest).  When every padded axis is a power of
+// two the solves run the pruned LDS transforms of bsn_fft.hip, which touch
+// only the octant that holds density; otherwise they use the hipFFT plans.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
+#include <cmath>
+#include <vector>
 #include "bsn_internal.hpp"
 #include "bsn_physics.hpp"
 #include "bsn_poisson.hpp"
@@ -138,6 +142,37 @@ hipError_t poisson_create(PoissonCtx& P, int nx, int ny, int nz, double dx, doub
                        nz, dx, dy, dz, eps2);
     HIPRET(hipGetLastError());
     FFTCHK(hipfftExecD2Z(P.fwd, P.pad, P.kspec));
+    if (fft_geom(nx, ny, nz, P.geom)) {
+        const FftGeom& F = P.geom;
+        const int64_t H = F.N[0] / 2 + 1;
+        HIPRET(hipMalloc((void**)&P.A, (size_t)nz * ny * H * sizeof(double2)));
+        HIPRET(hipMalloc((void**)&P.B, (size_t)nz * F.N[1] * H * sizeof(double2)));
+        HIPRET(hipMalloc((void**)&P.Kc, (size_t)(F.N[2] / 2 + 1) * (F.N[1] / 2 + 1) * H * sizeof(double)));
+        for (int a = 0; a < 3; ++a) {
+            const int N = F.N[a];
+            std::vector<double2> t(N / 2);
+            for (int p = 0; p < N / 2; ++p) {
+                const double ang = 2.0 * M_PI * (double)p / (double)N;
+                t[p] = make_double2(std::cos(ang), -std::sin(ang));
+            }
+            HIPRET(hipMalloc((void**)&P.tw[a], t.size() * sizeof(double2)));
+            HIPRET(hipMemcpyAsync(P.tw[a], t.data(), t.size() * sizeof(double2),
+                                  hipMemcpyHostToDevice, stream));
+            HIPRET(hipStreamSynchronize(stream));
+        }
+        HIPRET(fft_pack_kernel(stream, F, (const double2*)P.kspec, P.Kc));
+        HIPRET(hipStreamSynchronize(stream));
+        // the solves need neither the plans nor the padded arrays any more
+        (void)hipfftDestroy(P.fwd);
+        (void)hipfftDestroy(P.inv);
+        P.fwd = P.inv = 0;
+        HIPRET(hipFree(P.pad));
+        HIPRET(hipFree(P.spec));
+        HIPRET(hipFree(P.kspec));
+        HIPRET(hipFree(P.work));
+        P.pad = nullptr; P.spec = P.kspec = nullptr; P.work = nullptr;
+        P.pruned = true;
+    }
     return hipStreamSynchronize(stream);
 }
 
@@ -148,6 +183,10 @@ void poisson_destroy(PoissonCtx& P) {
     (void)hipFree(P.spec);
     (void)hipFree(P.kspec);
     (void)hipFree(P.work);
+    (void)hipFree(P.A);
+    (void)hipFree(P.B);
+    (void)hipFree(P.Kc);
+    for (int a = 0; a < 3; ++a) (void)hipFree(P.tw[a]);
     P = PoissonCtx{};
 }
 
@@ -166,8 +205,15 @@ static hipError_t poisson_convolve(PoissonCtx& P, double* phi, hipStream_t strea
     return hipGetLastError();
 }
 
+static double conv_scale(const PoissonCtx& P) {
+    return (P.dx * P.dy * P.dz) / (double)((int64_t)8 * P.nx * P.ny * P.nz);
+}
+
 hipError_t poisson_solve_state(PoissonCtx& P, const GridDev& G, int floors, double* phi,
                                hipStream_t stream) {
+    if (P.pruned)
+        return fft_convolve(stream, P.geom, G, nullptr, floors, P.A, P.B, P.Kc, P.tw, phi,
+                            conv_scale(P));
     const int64_t nreal = (int64_t)8 * P.nx * P.ny * P.nz;
     hipLaunchKernelGGL(k_pad_state, dim3(blocks_for(nreal, P.cus)), dim3(256), 0, stream, G, P.pad,
                        floors);
@@ -176,6 +222,10 @@ hipError_t poisson_solve_state(PoissonCtx& P, const GridDev& G, int floors, doub
 }
 
 hipError_t poisson_solve_array(PoissonCtx& P, const double* rho, double* phi, hipStream_t stream) {
+    if (P.pruned) {
+        GridDev none{};
+        return fft_convolve(stream, P.geom, none, rho, 0, P.A, P.B, P.Kc, P.tw, phi, conv_scale(P));
+    }
     const int64_t nreal = (int64_t)8 * P.nx * P.ny * P.nz;
     hipLaunchKernelGGL(k_pad_array, dim3(blocks_for(nreal, P.cus)), dim3(256), 0, stream, rho,
                        P.pad, P.nx, P.ny, P.nz);

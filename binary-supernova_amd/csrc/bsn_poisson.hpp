@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 #include "bsn_internal.hpp"
+#include "bsn_fft.hpp"
 
 namespace bsn {
 
@@ -16,6 +17,14 @@ struct PoissonCtx {
     int64_t ncplx = 0;
     void* work = nullptr;
     size_t work_bytes = 0;
+    // pruned LDS transforms when every padded axis is a power of two <= 2048
+    // (the hipFFT plans then only form the kernel spectrum at set-up)
+    bool pruned = false;
+    FftGeom geom{};
+    double2* A = nullptr;                   // nz*ny*(Nx/2+1) complex
+    double2* B = nullptr;                   // nz*Ny*(Nx/2+1) complex
+    double* Kc = nullptr;                   // real kernel spectrum, kz <= Nz/2, ky <= Ny/2
+    double2* tw[3] = {nullptr, nullptr, nullptr};
 };
 
 hipError_t poisson_create(PoissonCtx& P, int nx, int ny, int nz, double dx, double dy, double dz,
