@@ -260,8 +260,11 @@ __device__ __forceinline__ double cell_pressure(double r, double mx, double my, 
     return jl_max((g - 1.0) * (E - KE), 0.0);
 }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ float fmaxj(float x, float y) { return x > y ? x : y; }
-__device__ __forceinline__ float fminj(float x, float y) { return x < y ? x : y; }
+// Hardware max / min (one v_max_f32 / v_min_f32 instead of a compare and a
+// select).  They differ from the reference's max / min only in which operand
+// a NaN yields, which the split form does not promise to reproduce.
+__device__ __forceinline__ float fmaxj(float x, float y) { return __builtin_fmaxf(x, y); }
+__device__ __forceinline__ float fminj(float x, float y) { return __builtin_fminf(x, y); }
 
 // WENO5-Z weights.  eps is the reference's 1e-36 (a normal FP32 number),
 // raised to 1e-17*tau where needed so that (tau/(eps+beta))^2 stays below
