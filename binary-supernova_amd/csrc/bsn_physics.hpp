@@ -255,9 +255,9 @@ __device__ __forceinline__ double rcp64(double x) {
 // so that E - rho v^2 / 2 does not cancel in FP32.
 __device__ __forceinline__ double cell_pressure(double r, double mx, double my, double mz, double E,
                                                 double g) {
-    r = jl_max(r, 1e-30);
+    r = __builtin_fmax(r, 1e-30);
     const double KE = 0.5 * (mx * mx + my * my + mz * mz) * rcp64(r);
-    return jl_max((g - 1.0) * (E - KE), 0.0);
+    return __builtin_fmax((g - 1.0) * (E - KE), 0.0);
 }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 // Hardware max / min (one v_max_f32 / v_min_f32 instead of a compare and a
