@@ -504,7 +504,7 @@ def bench_selfgrav(bsn, args):
                              f"CFL 0.4, device dt", grid=[n, n, n], precision="split"),
         fp64=dict(value=cells * steps / r64["wall"], ms_per_step=1e3 * r64["wall"] / steps),
         poisson=dict(ms_per_solve=(r32["ms_step"] - r32["ms_plain"]) / 3,
-                     fft=(f"{2 * n}^3 zero-padded FP64 convolution: pruned radix-4 LDS "
+                     fft=(f"{2 * n}^3 zero-padded FP64 convolution: pruned register four-step "
                           f"transforms for power-of-two padded axes, hipFFT otherwise"),
                      step_share=(r32["ms_step"] - r32["ms_plain"]) / r32["ms_step"],
                      ms_step_without=r32["ms_plain"],

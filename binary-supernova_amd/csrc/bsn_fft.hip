@@ -1,6 +1,6 @@
 // bsn_fft.hip -- the zero-padded FFT convolution of the isolated Poisson solve
 // (solve_poisson_isolated, src/self_gravity.jl:35-76) as five pruned passes
-// over power-of-two padded axes, FP64 complex, in LDS:
+// over power-of-two padded axes, FP64 complex:
 //
 //   X  forward along x of the ny*nz rows that hold density (the other rows of
 //      the padded array are zero), real input read straight from the grid
@@ -12,12 +12,15 @@
 //   X' inverse along x from the Hermitian half spectrum, only i < nx kept,
 //      scaled by dV / (Nx Ny Nz):                          phi[k][j][i]
 //
-// Only the octant the solve reads or writes ever reaches HBM.  Each block
-// holds 2048 complex values in LDS: R = 2048 / N rows (x passes) or TW = 2048 /
-// N columns of consecutive m (y, z passes), transformed by radix-4 stages
-// (pairs of radix-2 stages): decimation in time on bit-reversed input for the
-// forward transforms, decimation in frequency (bit-reversed output) for the
-// inverse ones.
+// Only the octant the solve reads or writes ever reaches HBM.  A pass whose
+// padded length N = P * Q has Q = P or 2P and 16 <= N <= 256 runs the register
+// four-step form: each transform lives in the registers of P threads (Q values
+// each), one LDS exchange between a Q-point and a P-point DFT.  Other lengths
+// hold 2048 complex values per block in LDS: R = 2048 / N rows (x passes) or
+// TW = 2048 / N columns of consecutive m (y, z passes), transformed by radix-4
+// stages (pairs of radix-2 stages): decimation in time on bit-reversed input
+// for the forward transforms, decimation in frequency (bit-reversed output)
+// for the inverse ones.
 #include <hip/hip_runtime.h>
 #include "bsn_internal.hpp"
 #include "bsn_fft.hpp"
@@ -119,6 +122,303 @@ __device__ void fft_lds(double2* buf, const double2* tw, int logN, int lc) {
             done += 1;
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident transforms (N = P * Q with Q = P or 2P, 16 <= N <= 256).  A length-N transform is done by P threads, each
+// holding Q values: thread p holds element p + P*q in register q.  Four steps:
+// a length-Q DFT in registers, twiddles W_N^(p s), one transpose through LDS,
+// length-P DFTs in registers.  On exit register r*P + k1 of thread p holds
+// element (p + P*r) + Q*k1, which reordered (register r + (Q/P)*k1) is again
+// "element p + P*q in register q": a forward transform feeds the inverse one
+// without another transpose.
+
+static constexpr double W32R[16] = {1.0, 0.9807852804032304, 0.9238795325112867, 0.8314696123025452,
+                                   0.7071067811865476, 0.5555702330196023, 0.38268343236508984,
+                                   0.19509032201612833, 0.0, -0.1950903220161282, -0.3826834323650897,
+                                   -0.555570233019602, -0.7071067811865475, -0.8314696123025453,
+                                   -0.9238795325112867, -0.9807852804032304};
+static constexpr double W32I[16] = {0.0, -0.19509032201612825, -0.3826834323650898, -0.5555702330196022,
+                                   -0.7071067811865475, -0.8314696123025452, -0.9238795325112867,
+                                   -0.9807852804032304, -1.0, -0.9807852804032304, -0.9238795325112867,
+                                   -0.8314696123025455, -0.7071067811865476, -0.5555702330196022,
+                                   -0.3826834323650899, -0.1950903220161286};
+
+constexpr int brev_c(int i, int L) {
+    int r = 0;
+    for (int b = 1; b < L; b <<= 1, i >>= 1) r = (r << 1) | (i & 1);
+    return r;
+}
+
+// w * v for the twiddle W_32^k (conjugated for INV), k a compile-time index
+template <int K, bool INV>
+__device__ __forceinline__ double2 tw32(double2 v) {
+    if constexpr (K == 0) {
+        return v;
+    } else if constexpr (K == 8) {
+        return INV ? make_double2(-v.y, v.x) : make_double2(v.y, -v.x);     // +i / -i
+    } else {
+        const double wr = W32R[K], wi = INV ? -W32I[K] : W32I[K];
+        return make_double2(v.x * wr - v.y * wi, v.x * wi + v.y * wr);
+    }
+}
+
+template <int L, bool INV, int HALF, int POS>
+__device__ __forceinline__ void bfly_reg(double2* a) {
+#pragma unroll
+    for (int g = 0; g < L; g += 2 * HALF) {
+        const double2 t = tw32<POS * (32 / (2 * HALF)), INV>(a[g + POS + HALF]);
+        const double2 u = a[g + POS];
+        a[g + POS] = make_double2(u.x + t.x, u.y + t.y);
+        a[g + POS + HALF] = make_double2(u.x - t.x, u.y - t.y);
+    }
+}
+
+template <int L, bool INV, int HALF, int POS = 0>
+__device__ __forceinline__ void stage_reg(double2* a) {
+    bfly_reg<L, INV, HALF, POS>(a);
+    if constexpr (POS + 1 < HALF) stage_reg<L, INV, HALF, POS + 1>(a);
+}
+
+template <int L, bool INV, int HALF = 1>
+__device__ __forceinline__ void stages_reg(double2* a) {
+    stage_reg<L, INV, HALF>(a);
+    if constexpr (2 * HALF < L) stages_reg<L, INV, 2 * HALF>(a);
+}
+
+// In-place DFT of a[0..L-1] (natural order in and out), L <= 32, radix-2 with
+// constant twiddles.
+template <int L, bool INV>
+__device__ __forceinline__ void dft_reg(double2* a) {
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        const int j = brev_c(i, L);
+        if (j > i) { const double2 t = a[i]; a[i] = a[j]; a[j] = t; }
+    }
+    stages_reg<L, INV>(a);
+}
+
+// W_N^m for 0 <= m < N from the table of W_N^p, p < N/2
+template <bool INV>
+__device__ __forceinline__ double2 twN(const double2* tw, int m, int N) {
+    double2 w = tw[m & ((N >> 1) - 1)];
+    if (m >= (N >> 1)) { w.x = -w.x; w.y = -w.y; }
+    if (INV) w.y = -w.y;
+    return w;
+}
+
+// The four steps on x (thread p, transform c of ncol); lds holds N * ncol
+// values, transform-interleaved (ROWS = false: value i of transform c at
+// i * ncol + c, for threads with c fastest) or transform by transform
+// (ROWS = true: c * (N + Q) + i + i / P, for threads with p fastest; the pad
+// of one slot per P keeps the strided reads of the second step off a single
+// bank).
+template <int P, int Q, bool INV, bool ROWS = false>
+__device__ __forceinline__ void fft_reg(double2 (&x)[Q], int p, int c, int ncol, double2* lds,
+                                        const double2* tw) {
+    constexpr int N = P * Q;
+    auto at = [&](int i) -> double2& {
+        return ROWS ? lds[c * (N + Q) + i + i / P] : lds[i * ncol + c];
+    };
+    dft_reg<Q, INV>(x);
+#pragma unroll
+    for (int s = 0; s < Q; ++s) x[s] = cmul(x[s], twN<INV>(tw, p * s, N));
+    __syncthreads();                       // lds free (a previous transform read it)
+#pragma unroll
+    for (int s = 0; s < Q; ++s) at(s * P + p) = x[s];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < Q / P; ++r) {
+        const int s = p + P * r;
+#pragma unroll
+        for (int k = 0; k < P; ++k) x[r * P + k] = at(s * P + k);
+        dft_reg<P, INV>(x + r * P);
+    }
+}
+
+// Element index held by register j after fft_reg, and the reordering that
+// makes the output the next transform's input.
+template <int P, int Q>
+__device__ __forceinline__ int reg_elem(int p, int j) {
+    const int r = j / P, k1 = j % P;
+    return (p + P * r) + Q * k1;
+}
+template <int P, int Q>
+__device__ __forceinline__ void reg_to_input(double2 (&x)[Q]) {
+    double2 t[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) t[(j / P) + (Q / P) * (j % P)] = x[j];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) x[j] = t[j];
+}
+
+// Y forward, register form: block (m-range, k), 256 threads = (256/P columns) x P.
+template <int P, int Q>
+__global__ void __launch_bounds__(256) k_fftr_y_fwd(FftGeom F, const double2* __restrict__ A,
+                                                    double2* __restrict__ B,
+                                                    const double2* __restrict__ twy) {
+    constexpr int N = P * Q, TW = 256 / P;
+    extern __shared__ double2 lds[];                  // N * TW values, then N / 2 twiddles
+    double2* tw = lds + N * TW;
+    const int H = (F.N[0] >> 1) + 1, ny = F.n[1];
+    const int c = threadIdx.x % TW, p = threadIdx.x / TW;
+    const int m = blockIdx.x * TW + c, k = blockIdx.y;
+    load_tw(tw, twy, N);
+    double2 x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = p + P * q;
+        x[q] = (q < Q / 2 && e < ny && m < H) ? A[((int64_t)k * ny + e) * H + m] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_reg<P, Q, false>(x, p, c, TW, lds, tw);
+    if (m < H) {
+#pragma unroll
+        for (int j = 0; j < Q; ++j) B[((int64_t)k * N + reg_elem<P, Q>(p, j)) * H + m] = x[j];
+    }
+}
+
+// Z, register form: forward, kernel spectrum, inverse; block (m-range, jj).
+template <int P, int Q>
+__global__ void __launch_bounds__(256) k_fftr_z(FftGeom F, double2* __restrict__ B,
+                                                const double* __restrict__ Kc,
+                                                const double2* __restrict__ twz) {
+    constexpr int N = P * Q, TW = 256 / P;
+    extern __shared__ double2 lds[];
+    double2* tw = lds + N * TW;
+    const int H = (F.N[0] >> 1) + 1, Ny = F.N[1], nz = F.n[2];
+    const int c = threadIdx.x % TW, p = threadIdx.x / TW;
+    const int m = blockIdx.x * TW + c, jj = blockIdx.y;
+    load_tw(tw, twz, N);
+    double2 x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = p + P * q;
+        x[q] = (q < Q / 2 && e < nz && m < H) ? B[((int64_t)e * Ny + jj) * H + m] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_reg<P, Q, false>(x, p, c, TW, lds, tw);
+    const int ky = min(jj, Ny - jj);
+    const int mm = m < H ? m : H - 1;
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+        const int e = reg_elem<P, Q>(p, j), kz = min(e, N - e);
+        const double kv = Kc[((int64_t)kz * ((Ny >> 1) + 1) + ky) * H + mm];
+        x[j] = make_double2(x[j].x * kv, x[j].y * kv);
+    }
+    reg_to_input<P, Q>(x);
+    fft_reg<P, Q, true>(x, p, c, TW, lds, tw);
+    if (m < H) {
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+            const int e = reg_elem<P, Q>(p, j);
+            if (e < nz) B[((int64_t)e * Ny + jj) * H + m] = x[j];
+        }
+    }
+}
+
+// Y inverse, register form: block (m-range, k); only j < ny kept.
+template <int P, int Q>
+__global__ void __launch_bounds__(256) k_fftr_y_inv(FftGeom F, const double2* __restrict__ B,
+                                                    double2* __restrict__ A,
+                                                    const double2* __restrict__ twy) {
+    constexpr int N = P * Q, TW = 256 / P;
+    extern __shared__ double2 lds[];
+    double2* tw = lds + N * TW;
+    const int H = (F.N[0] >> 1) + 1, ny = F.n[1];
+    const int c = threadIdx.x % TW, p = threadIdx.x / TW;
+    const int m = blockIdx.x * TW + c, k = blockIdx.y;
+    load_tw(tw, twy, N);
+    double2 x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        x[q] = m < H ? B[((int64_t)k * N + p + P * q) * H + m] : make_double2(0.0, 0.0);
+    __syncthreads();
+    fft_reg<P, Q, true>(x, p, c, TW, lds, tw);
+    if (m < H) {
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+            const int e = reg_elem<P, Q>(p, j);
+            if (e < ny) A[((int64_t)k * ny + e) * H + m] = x[j];
+        }
+    }
+}
+
+// X forward, register form: 256 threads = (256/P rows) x P, p fastest so that
+// consecutive threads read consecutive cells of a row.
+template <int P, int Q>
+__global__ void __launch_bounds__(256) k_fftr_x_fwd(GridDev G, const double* __restrict__ rho,
+                                                    int floors, FftGeom F, double2* __restrict__ A,
+                                                    const double2* __restrict__ twx) {
+    constexpr int N = P * Q, TR = 256 / P;
+    extern __shared__ double2 lds[];
+    double2* tw = lds + (N + Q) * TR;
+    const int H = (N >> 1) + 1, nx = F.n[0], rows = F.n[1] * F.n[2];
+    const int p = threadIdx.x % P, c = threadIdx.x / P;
+    const int row = blockIdx.x * TR + c;
+    const int ly = F.logN[1] - 1;                     // ny = Ny / 2
+    load_tw(tw, twx, N);
+    double2 x[Q];
+    const int j = row & ((1 << ly) - 1), k = row >> ly;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = p + P * q;
+        double v = 0.0;
+        if (q < Q / 2 && e < nx && row < rows) {
+            if (rho) {
+                v = rho[(int64_t)row * nx + e];
+            } else {
+                v = G.U[((int64_t)(NG + k) * G.Ny + NG + j) * G.pitch + BSN_XOFF + NG + e];
+                if (floors && v < G.rho_floor) v = G.rho_floor;   // src/gpu_kernels.jl:26-46
+            }
+        }
+        x[q] = make_double2(v, 0.0);
+    }
+    __syncthreads();
+    fft_reg<P, Q, false, true>(x, p, c, TR, lds, tw);
+    if (row < rows) {
+#pragma unroll
+        for (int jx = 0; jx < Q; ++jx) {
+            const int e = reg_elem<P, Q>(p, jx);
+            if (e < H) A[(int64_t)row * H + e] = x[jx];
+        }
+    }
+}
+
+// X inverse, register form: the full spectrum from its Hermitian half, only
+// i < nx kept, scaled, into phi.
+template <int P, int Q>
+__global__ void __launch_bounds__(256) k_fftr_x_inv(FftGeom F, const double2* __restrict__ A,
+                                                    double* __restrict__ phi, double scale,
+                                                    const double2* __restrict__ twx) {
+    constexpr int N = P * Q, TR = 256 / P;
+    extern __shared__ double2 lds[];
+    double2* tw = lds + (N + Q) * TR;
+    const int H = (N >> 1) + 1, nx = F.n[0], rows = F.n[1] * F.n[2];
+    const int p = threadIdx.x % P, c = threadIdx.x / P;
+    const int row = blockIdx.x * TR + c;
+    load_tw(tw, twx, N);
+    double2 x[Q];
+    const double2* a = A + (int64_t)(row < rows ? row : 0) * H;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = p + P * q;
+        double2 v = make_double2(0.0, 0.0);
+        if (row < rows) {
+            if (e < H) v = a[e];
+            else { v = a[N - e]; v.y = -v.y; }
+        }
+        x[q] = v;
+    }
+    __syncthreads();
+    fft_reg<P, Q, true, true>(x, p, c, TR, lds, tw);
+    if (row < rows) {
+#pragma unroll
+        for (int jx = 0; jx < Q; ++jx) {
+            const int e = reg_elem<P, Q>(p, jx);
+            if (e < nx) phi[(int64_t)row * nx + e] = x[jx].x * scale;
+        }
     }
 }
 
@@ -315,6 +615,61 @@ hipError_t fft_pack_kernel(hipStream_t s, const FftGeom& F, const double2* kspec
     return hipGetLastError();
 }
 
+// Each pass in register form when its axis has N = P*Q with Q in {P, 2P} and
+// 16 <= N <= 256, else in the LDS radix-4 form.
+enum { PASS_X_FWD, PASS_Y_FWD, PASS_Z, PASS_Y_INV, PASS_X_INV };
+
+struct RegArgs {
+    const GridDev* G;
+    const double* rho;
+    int floors;
+    double2* A;
+    double2* B;
+    const double* Kc;
+    const double2* tw;
+    double* phi;
+    double scale;
+};
+
+template <int P, int Q>
+static void launch_reg(int pass, hipStream_t s, const FftGeom& F, const RegArgs& a) {
+    constexpr int N = P * Q, TW = 256 / P;
+    const int H = (F.N[0] >> 1) + 1, rows = F.n[1] * F.n[2];
+    const size_t lds = (size_t)((N + Q) * TW + N / 2) * sizeof(double2);
+    const dim3 gy((H + TW - 1) / TW, pass == PASS_Z ? F.N[1] : F.n[2]);
+    const dim3 gx((rows + TW - 1) / TW);
+    switch (pass) {
+        case PASS_X_FWD:
+            hipLaunchKernelGGL((k_fftr_x_fwd<P, Q>), gx, dim3(256), lds, s, *a.G, a.rho, a.floors, F,
+                               a.A, a.tw);
+            break;
+        case PASS_Y_FWD:
+            hipLaunchKernelGGL((k_fftr_y_fwd<P, Q>), gy, dim3(256), lds, s, F, a.A, a.B, a.tw);
+            break;
+        case PASS_Z:
+            hipLaunchKernelGGL((k_fftr_z<P, Q>), gy, dim3(256), lds, s, F, a.B, a.Kc, a.tw);
+            break;
+        case PASS_Y_INV:
+            hipLaunchKernelGGL((k_fftr_y_inv<P, Q>), gy, dim3(256), lds, s, F, a.B, a.A, a.tw);
+            break;
+        default:
+            hipLaunchKernelGGL((k_fftr_x_inv<P, Q>), gx, dim3(256), lds, s, F, a.A, a.phi, a.scale,
+                               a.tw);
+            break;
+    }
+}
+
+static bool launch_reg_any(int pass, int N, hipStream_t s, const FftGeom& F, const RegArgs& a) {
+    switch (N) {
+        case 16:   launch_reg<4, 4>(pass, s, F, a); return true;
+        case 32:   launch_reg<4, 8>(pass, s, F, a); return true;
+        case 64:   launch_reg<8, 8>(pass, s, F, a); return true;
+        case 128:  launch_reg<8, 16>(pass, s, F, a); return true;
+        case 256:  launch_reg<16, 16>(pass, s, F, a); return true;
+        default:   return false;     // longer transforms would hold too many registers
+    }
+}
+
 hipError_t fft_convolve(hipStream_t s, const FftGeom& F, const GridDev& G, const double* rho,
                         int floors, double2* A, double2* B, const double* Kc,
                         const double2* const tw[3], double* phi, double scale) {
@@ -323,16 +678,27 @@ hipError_t fft_convolve(hipStream_t s, const FftGeom& F, const GridDev& G, const
     const int Rx = FFT_CE / F.N[0];
     const int TWy = FFT_CE / F.N[1], TWz = FFT_CE / F.N[2];
     auto lds = [](int N) { return (size_t)(FFT_CE + N / 2) * sizeof(double2); };
-    hipLaunchKernelGGL(k_fft_x_fwd, dim3((rows + Rx - 1) / Rx), dim3(256), lds(F.N[0]), s, G, rho,
-                       floors, F, A, tw[0]);
-    hipLaunchKernelGGL(k_fft_y_fwd, dim3((H + TWy - 1) / TWy, F.n[2]), dim3(256), lds(F.N[1]), s, F,
-                       A, B, tw[1]);
-    hipLaunchKernelGGL(k_fft_z, dim3((H + TWz - 1) / TWz, F.N[1]), dim3(256), lds(F.N[2]), s, F, B,
-                       Kc, tw[2]);
-    hipLaunchKernelGGL(k_fft_y_inv, dim3((H + TWy - 1) / TWy, F.n[2]), dim3(256), lds(F.N[1]), s, F,
-                       B, A, tw[1]);
-    hipLaunchKernelGGL(k_fft_x_inv, dim3((rows + Rx - 1) / Rx), dim3(256), lds(F.N[0]), s, F, A, phi,
-                       scale, tw[0]);
+    RegArgs a{&G, rho, floors, A, B, Kc, nullptr, phi, scale};
+    a.tw = tw[0];
+    if (!launch_reg_any(PASS_X_FWD, F.N[0], s, F, a))
+        hipLaunchKernelGGL(k_fft_x_fwd, dim3((rows + Rx - 1) / Rx), dim3(256), lds(F.N[0]), s, G, rho,
+                           floors, F, A, tw[0]);
+    a.tw = tw[1];
+    if (!launch_reg_any(PASS_Y_FWD, F.N[1], s, F, a))
+        hipLaunchKernelGGL(k_fft_y_fwd, dim3((H + TWy - 1) / TWy, F.n[2]), dim3(256), lds(F.N[1]), s,
+                           F, A, B, tw[1]);
+    a.tw = tw[2];
+    if (!launch_reg_any(PASS_Z, F.N[2], s, F, a))
+        hipLaunchKernelGGL(k_fft_z, dim3((H + TWz - 1) / TWz, F.N[1]), dim3(256), lds(F.N[2]), s, F,
+                           B, Kc, tw[2]);
+    a.tw = tw[1];
+    if (!launch_reg_any(PASS_Y_INV, F.N[1], s, F, a))
+        hipLaunchKernelGGL(k_fft_y_inv, dim3((H + TWy - 1) / TWy, F.n[2]), dim3(256), lds(F.N[1]), s,
+                           F, B, A, tw[1]);
+    a.tw = tw[0];
+    if (!launch_reg_any(PASS_X_INV, F.N[0], s, F, a))
+        hipLaunchKernelGGL(k_fft_x_inv, dim3((rows + Rx - 1) / Rx), dim3(256), lds(F.N[0]), s, F, A,
+                           phi, scale, tw[0]);
     return hipGetLastError();
 }
 

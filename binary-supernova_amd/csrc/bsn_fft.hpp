@@ -1,4 +1,4 @@
-// bsn_fft.hpp -- pruned LDS FFT convolution for the isolated Poisson solve
+// bsn_fft.hpp -- pruned FFT convolution for the isolated Poisson solve
 // on power-of-two padded axes (bsn_fft.hip).
 #pragma once
 #include <hip/hip_runtime.h>

@@ -7,7 +7,7 @@
 // octant of dV * IFFT(FFT(K) FFT(rho_pad)) is the potential.  The kernel
 // spectrum is computed once per grid and cell size with real-to-complex hipFFT
 // plans of the padded array (x fastest).  When every padded axis is a power of
-// two the solves run the pruned LDS transforms of bsn_fft.hip, which touch
+// two the solves run the pruned transforms of bsn_fft.hip, which touch
 // only the octant that holds density; otherwise they use the hipFFT plans.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>

@@ -17,7 +17,7 @@ struct PoissonCtx {
     int64_t ncplx = 0;
     void* work = nullptr;
     size_t work_bytes = 0;
-    // pruned LDS transforms when every padded axis is a power of two <= 2048
+    // pruned transforms when every padded axis is a power of two <= 2048
     // (the hipFFT plans then only form the kernel spectrum at set-up)
     bool pruned = false;
     FftGeom geom{};

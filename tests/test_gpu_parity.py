@@ -523,8 +523,13 @@ def test_poisson_solve_matches_oracle_and_reference_checks():
     from oracle import poisson as PO
     print("poisson", flush=True)
     rng = np.random.default_rng(81)
-    # power-of-two padded axes run the pruned LDS transforms, others hipFFT
+    # power-of-two padded axes run the pruned transforms (register four-step
+    # forms for padded lengths 16..256, radix-4 LDS stages otherwise), other
+    # grids hipFFT; the shapes give each pass padded lengths 8 to 512
     for (nx, ny, nz), d in (((32, 32, 32), (1 / 32,) * 3), ((64, 16, 32), (0.05, 0.1, 0.07)),
+                            ((128, 64, 8), (0.01, 0.02, 0.03)), ((8, 16, 128), (0.03, 0.02, 0.01)),
+                            ((8, 128, 64), (0.03, 0.01, 0.02)), ((256, 4, 4), (0.004, 0.3, 0.2)),
+                            ((4, 256, 8), (0.2, 0.004, 0.1)),
                             ((20, 14, 9), (0.1, 0.07, 0.12))):
         rho = rng.uniform(0.0, 2.0, (nz, ny, nx))
         phi = bsn.solve_poisson_isolated(rho, nx, ny, nz, *d)
