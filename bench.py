@@ -170,6 +170,29 @@ def roofline(res, prec, interfaces_per_sweep, cells, n=None):
     return flux, upd
 
 
+def standalone_update_roofline(res, prec, cells, steps=10):
+    """The stand-alone update kernel, k_update: divergence of the three
+    materialised flux arrays, RK3 stage, floors and CFL, in place (the step
+    form of the FMR levels).  Timed with HIP events in the in-place form of
+    the same batch and state.  Algorithmic bytes per cell, averaged over the
+    stages: U read and written (80), the x, y and z fluxes (3 x 5 words) and
+    the base state in stages 2 and 3 (2/3 x 40)."""
+    b = res["batch"]
+    b.set_mode(True)
+    try:
+        _, ms, _ = b.profile(steps)
+    finally:
+        b.set_mode(False)
+    w = 8 if prec == "fp64" else 4
+    per_cell = 2 * 5 * 8 + 3 * 5 * w + (2.0 / 3.0) * 5 * 8
+    nbytes = per_cell * cells
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return dict(kernel="k_update (in-place form: flux divergence of three flux arrays + RK3 stage "
+                       "+ floors + CFL)", bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=gbs / HBM_PEAK_GBS, algorithmic_bytes=nbytes,
+                bytes_per_cell=per_cell, ms_per_launch=ms, steps=steps)
+
+
 def step_roofline(zone_updates_per_s, bytes_per_zone_update=320):
     """Whole-step HBM position: 320 B is the algorithmic FP64-state traffic of
     one SSP-RK3 step with perfect on-chip stencil reuse (stage 1 reads U and
@@ -230,6 +253,9 @@ def bench_turb(bsn, args):
         copy_calibration_bytes=copy_bytes,
         roofline_step=step_roofline(v32),
     )
+    log("stand-alone update kernel (in-place form)")
+    out["roofline_update"] = standalone_update_roofline(r32, "split", cells)
+    out["fp64"]["roofline_update"] = standalone_update_roofline(r64, "fp64", cells)
     out["pcie_inclusive"] = host_array_rate(bsn, U, dims, d, gamma, r32["tune"])
     if not args.no_cpu:
         log("cpu baseline")

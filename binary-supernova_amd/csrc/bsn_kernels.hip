@@ -34,6 +34,16 @@ __device__ __forceinline__ int64_t uidx(const GridDev& G, int i, int j, int k) {
     return ((int64_t)k * G.Ny + j) * G.pitch + BSN_XOFF + i;
 }
 
+// Global-address-space view of a device pointer.  The state and flux
+// pointers come out of the GridDev table in memory, so the compiler cannot
+// infer their address space and would emit flat loads and stores; flat
+// operations also count against the scalar-load counter, and every scalar
+// reload of a table field would then wait for all HBM loads in flight.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ void chunk(int ntiles, int& t0, int& t1) {
     t0 = (int)(((int64_t)blockIdx.x * ntiles) / gridDim.x);
     t1 = (int)(((int64_t)(blockIdx.x + 1) * ntiles) / gridDim.x);
@@ -179,7 +189,7 @@ __device__ __forceinline__ void load_cell(const GridDev& G, int i, int j, int k,
     }
     const int64_t o = uidx(G, si, sj, sk);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) u[q] = G.U[q * G.fs + o];
+    for (int q = 0; q < 5; ++q) u[q] = gmem(G.U)[q * G.fs + o];
     if (FLOOR && (G.lmode & 2) && si == i && sj == j && sk == k)
         apply_floor<FAST>(u, G.rho_floor, G.P_floor, G.gamma);
 }
@@ -262,7 +272,7 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
     auto emit = [&](const GridDev& G, const T (&c)[4][6], const T (&p)[6], int f, int ci, int cj) {
         T F[5];
         Prec<T>::template flux<DIR>(c, p, G, F);
-        T* Fp = (T*)G.F[DIR];
+        auto Fp = gmem((T*)G.F[DIR]);
         int64_t o;
         if (DIR == 0)      o = ((int64_t)cj * G.ny + ci) * G.fp[0] + f;
         else if (DIR == 1) o = ((int64_t)cj * (G.ny + 1) + f) * G.fp[1] + ci;
@@ -458,8 +468,8 @@ __device__ __forceinline__ void cell_state(const GridDev& G, int64_t c, int stag
     const bool need_u = stage > 1 || G.phi != nullptr || G.src != nullptr;
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-        un[q] = G.Un[q * G.fs + c];
-        u[q] = need_u ? G.U[q * G.fs + c] : 0.0;
+        un[q] = gmem(G.Un)[q * G.fs + c];
+        u[q] = need_u ? gmem(G.U)[q * G.fs + c] : 0.0;
     }
     if (need_u && (G.lmode & 2)) apply_floor<FAST>(u, G.rho_floor, G.P_floor, G.gamma);
 }
@@ -473,7 +483,7 @@ __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, in
     // (scripts/run_sn50_fiducial.jl:772-781): point masses, self-gravity, sinks
     if (G.src && G.src->nbh > 0) bh_gravity_add(*G.src, i, j, k, u, d);
     if (G.phi) {
-        const double* ph = G.phi;
+        const auto ph = gmem(G.phi);
         const int ip = min(i + 1, G.nx - 1), im = max(i - 1, 0);
         const int jp = min(j + 1, G.ny - 1), jm = max(j - 1, 0);
         const int kp = min(k + 1, G.nz - 1), km = max(k - 1, 0);
@@ -504,16 +514,8 @@ __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, in
     }
     apply_floor<FAST>(v, G.rho_floor, G.P_floor, G.gamma);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) G.Uout[q * G.fs + c] = v[q];
+    for (int q = 0; q < 5; ++q) gmem(G.Uout)[q * G.fs + c] = v[q];
     return do_cfl ? cfl_speed_bits<FAST>(v, G) : 0;
-}
-
-template <bool FAST = false>
-__device__ __forceinline__ uint64_t finish_cell(const GridDev& G, int i, int j, int k, int stage,
-                                                double (&d)[5], int do_cfl) {
-    double u[5], un[5];
-    cell_state<FAST>(G, uidx(G, NG + i, NG + j, NG + k), stage, u, un);
-    return finish_cell_with<FAST>(G, i, j, k, stage, d, u, un, do_cfl, G.scal[SCAL_DT]);
 }
 
 template <typename T, int KZ>
@@ -529,10 +531,11 @@ __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grid
         const int i = tl.a + lx, j = tl.b + ly;
         uint64_t smax = 0;
         if (i < G.nx && j < G.ny) {
-            const T* Fx = (const T*)G.F[0];
-            const T* Fy = (const T*)G.F[1];
-            const T* Fz = (const T*)G.F[2];
+            const auto Fx = gmem((const T*)G.F[0]);
+            const auto Fy = gmem((const T*)G.F[1]);
+            const auto Fz = gmem((const T*)G.F[2]);
             const int kend = min(tl.c + KZ, G.nz);
+            const double dt = stage != 0 ? G.scal[SCAL_DT] : 0.0;
             double fz0[5];
             {
                 const int64_t o = ((int64_t)tl.c * G.ny + j) * G.fp[2] + i;
@@ -543,6 +546,11 @@ __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grid
                 const int64_t ox = ((int64_t)k * G.ny + j) * G.fp[0] + i;
                 const int64_t oy = ((int64_t)k * (G.ny + 1) + j) * G.fp[1] + i;
                 const int64_t oz = ((int64_t)(k + 1) * G.ny + j) * G.fp[2] + i;
+                const int64_t c = uidx(G, NG + i, NG + j, NG + k);
+                // the cell's state is loaded with its fluxes: one round trip
+                // to HBM per cell
+                double u[5], un[5];
+                if (stage != 0) cell_state<sizeof(T) == 4>(G, c, stage, u, un);
                 double d[5], fz1[5];
 #pragma unroll
                 for (int q = 0; q < 5; ++q) {
@@ -555,11 +563,11 @@ __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grid
                     d[q] = (-(fx1 - fx0)) * G.idx - (fy1 - fy0) * G.idy - (fz1[q] - fz0[q]) * G.idz;
                 }
                 if (stage == 0) {
-                    const int64_t c = uidx(G, NG + i, NG + j, NG + k);
 #pragma unroll
                     for (int q = 0; q < 5; ++q) dU_out[q * G.fs + c] = d[q];
                 } else {
-                    const uint64_t b = finish_cell<sizeof(T) == 4>(G, i, j, k, stage, d, do_cfl);
+                    const uint64_t b =
+                        finish_cell_with<sizeof(T) == 4>(G, i, j, k, stage, d, u, un, do_cfl, dt);
                     smax = b > smax ? b : smax;
                 }
 #pragma unroll
@@ -604,8 +612,8 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
         if (i < G.nx && j < G.ny) {
             const int kend = min(k0 + klen, G.nz);
             const double dt = G.scal[SCAL_DT];
-            const T* Fx = (const T*)G.F[0];
-            const T* Fy = (const T*)G.F[1];
+            const auto Fx = gmem((const T*)G.F[0]);
+            const auto Fy = gmem((const T*)G.F[1]);
             T c[4][6], p[6];
             // interface f (between active cells f-1 and f) uses padded z = f .. f+5
             const int64_t col = uidx(G, NG + i, NG + j, 0);
@@ -652,7 +660,7 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
                     for (int q = 0; q < 5; ++q) {
                         const double r = ring[sl][q][rt];
                         u[q] = r;
-                        un[q] = un_in_ring ? r : G.Un[q * G.fs + cc];
+                        un[q] = un_in_ring ? r : gmem(G.Un)[q * G.fs + cc];
                     }
                     if (need_u && (G.lmode & 2)) apply_floor<FAST>(u, G.rho_floor, G.P_floor, G.gamma);
                 } else {
