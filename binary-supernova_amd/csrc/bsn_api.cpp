@@ -80,6 +80,7 @@ struct bsn_grid {
     size_t stage_bytes = 0;
     double* d_part = nullptr;      // totals partial sums
     double* d_work = nullptr;      // third state buffer of the out-of-place step
+    double* d_work2 = nullptr;     // fourth state buffer (the fused FMR level step)
     SrcDev hs{};                   // point masses and sinks (host copy)
     SrcDev* d_src = nullptr;
     PoissonCtx* pois = nullptr;    // FFT Poisson solver (created on first use)
@@ -134,6 +135,8 @@ struct bsn_fmr {
     std::vector<double> alpha;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
+    int inplace = 0;                 // 1: levels step in place with all fluxes materialised
+    std::vector<GridDev*> d_fst;     // per level, the three stage tables of the fused step
 };
 
 // ---------------------------------------------------------------------------
@@ -250,6 +253,13 @@ static int ensure_work(bsn_grid* g) {
     if (g->d_work) return BSN_OK;
     HIPCHK(hipMalloc((void**)&g->d_work, 5 * g->h.fs * 8));
     HIPCHK(hipMemset(g->d_work, 0, 5 * g->h.fs * 8));
+    return BSN_OK;
+}
+
+static int ensure_work2(bsn_grid* g) {
+    if (g->d_work2) return BSN_OK;
+    HIPCHK(hipMalloc((void**)&g->d_work2, 5 * g->h.fs * 8));
+    HIPCHK(hipMemset(g->d_work2, 0, 5 * g->h.fs * 8));
     return BSN_OK;
 }
 
@@ -571,6 +581,7 @@ extern "C" int bsn_grid_destroy(bsn_grid* g) {
     (void)hipFree(g->d_stage);
     (void)hipFree(g->d_part);
     (void)hipFree(g->d_work);
+    (void)hipFree(g->d_work2);
     (void)hipFree(g->d_src);
     if (g->pois) { poisson_destroy(*g->pois); delete g->pois; }
     (void)hipFree(g->d_rho_in);
@@ -1571,6 +1582,21 @@ extern "C" int bsn_fmr_create(bsn_ctx* ctx, int nlevels, bsn_grid* const* levels
     m->nlev = nlevels;
     m->lev.assign(levels, levels + nlevels);
     m->alpha.assign(nlevels, 0.0);
+    // the z interfaces a level's flux registers read, which the fused step
+    // stores: a parent's faces at its child's box (interfaces ck_lo - 1 and
+    // ck_hi of k_fr_coarse), a child's outer faces (0 and nz of k_fr_fine)
+    for (int l = 0; l < nlevels; ++l) {
+        GridDev& h = levels[l]->h;
+        h.nzkeep = 0;
+        if (l + 1 < nlevels) {
+            h.zkeep[h.nzkeep++] = (int)lo_hi[6 * l + 4] - 1;
+            h.zkeep[h.nzkeep++] = (int)lo_hi[6 * l + 5];
+        }
+        if (l > 0) {
+            h.zkeep[h.nzkeep++] = 0;
+            h.zkeep[h.nzkeep++] = h.nz;
+        }
+    }
     auto fail = [&](int code) {
         bsn_fmr_destroy(m);
         return code;
@@ -1636,9 +1662,42 @@ extern "C" int bsn_fmr_destroy(bsn_fmr* m) {
     graph_drop(m->gexec, m->graph);
     for (auto* P : m->plans) plan_free(P);
     for (auto& f : m->hf) { (void)hipFree(f.frc); (void)hipFree(f.frf); (void)hipFree(f.Uci); }
+    for (auto* t : m->d_fst) (void)hipFree(t);
+    for (auto* g : m->lev) g->h.nzkeep = 0;
     (void)hipFree(m->d_f);
     (void)hipFree(m->d_all);
     delete m;
+    return BSN_OK;
+}
+
+// Stage tables of the fused level step.  Level buffers: U (the state), Un
+// (its raw copy at the start of the step, the RK3 base state and, for a
+// parent, the start state its child interpolates from), W1 and W2 (work):
+// stage 1 reads U and writes W1, stage 2 reads W1 and writes W2, stage 3
+// reads W2 and writes U, so U ends with the new state and Un keeps the old
+// one as in the in-place step.  The root maps ghost cells to active cells
+// under its boundary conditions on load; a child reads the ghost cells
+// prolonged into each stage's input buffer.  Stage 1 floors active cells on
+// load (the start-of-stage floors).
+static int fmr_stage_tables(bsn_fmr* m) {
+    if (m->d_fst.empty()) m->d_fst.assign(3 * m->nlev, nullptr);
+    for (int l = 0; l < m->nlev; ++l) {
+        bsn_grid* g = m->lev[l];
+        CHK(ensure_work(g));
+        CHK(ensure_work2(g));
+        const int map = l == 0 ? 1 : 0;
+        GridDev d = g->h;
+        d.Un = g->h.Un;
+        GridDev st[3] = {d, d, d};
+        st[0].lmode = map | 2; st[0].U = g->h.U;   st[0].Uout = g->d_work;
+        st[1].lmode = map;     st[1].U = g->d_work; st[1].Uout = g->d_work2;
+        st[2].lmode = map;     st[2].U = g->d_work2; st[2].Uout = g->h.U;
+        for (int s = 0; s < 3; ++s) {
+            GridDev*& t = m->d_fst[3 * l + s];
+            if (!t) HIPCHK(hipMalloc((void**)&t, sizeof(GridDev)));
+            HIPCHK(hipMemcpy(t, &st[s], sizeof(GridDev), hipMemcpyHostToDevice));
+        }
+    }
     return BSN_OK;
 }
 
@@ -1647,6 +1706,7 @@ static int fmr_refresh(bsn_fmr* m) {
     for (auto* g : m->lev) all.push_back(g->h);
     HIPCHK(hipMemcpy(m->d_all, all.data(), all.size() * sizeof(GridDev), hipMemcpyHostToDevice));
     for (auto* P : m->plans) CHK(plan_refresh(*P));
+    if (!m->inplace) CHK(fmr_stage_tables(m));
     return BSN_OK;
 }
 
@@ -1656,12 +1716,13 @@ static int64_t ghost_total(const GridDev& d) {
 
 // interp: from the parent state interpolated in time (fmr_interp), else from
 // the parent's current state.
-static int fmr_prolong(bsn_fmr* m, int l, int all_cells, bool interp) {
+static int fmr_prolong(bsn_fmr* m, int l, int all_cells, bool interp,
+                       ProlongDst dst = ProlongDst{0, {}}) {
     const GridDev& C = m->lev[l]->h;
     const GridDev& Pg = m->lev[l - 1]->h;
     const int64_t total = all_cells ? (int64_t)(C.nx + 2 * BSN_NG) * C.Ny * C.Nz : ghost_total(C);
     HIPCHK(launch_prolong(m->ctx->L(), m->d_all, m->d_f + (l - 1), total,
-                          interp ? m->hf[l - 1].Uci : Pg.U, all_cells));
+                          interp ? m->hf[l - 1].Uci : Pg.U, all_cells, dst));
     return BSN_OK;
 }
 
@@ -1685,6 +1746,29 @@ static int fmr_advance(bsn_fmr* m, int l) {
     const bool child = l + 1 < m->nlev;
     const GridDev& h = m->lev[l]->h;
     if (child) HIPCHK(launch_fr_zero(L, m->d_f + l, m->hf[l].froff[6]));
+    if (!m->inplace) {
+        // fused form: per stage the x and y sweeps, then the z sweep fused
+        // with the update (storing the z faces the registers read), then the
+        // flux-register sums of the stage; same bits as the in-place form
+        HIPCHK(launch_copy_state(L, h, h.U, h.Un));
+        if (l > 0) {
+            // alpha and the interpolated parent are fixed for the sub-cycle,
+            // so the ghost cells of all three stage inputs are prolonged once
+            CHK(fmr_interp(m, l));
+            bsn_grid* g = m->lev[l];
+            CHK(fmr_prolong(m, l, 0, true, ProlongDst{3, {g->h.U, g->d_work, g->d_work2}}));
+        }
+        for (int s = 1; s <= 3; ++s) {
+            const GridDev* T = m->d_fst[3 * l + s - 1];
+            CHK(plan_fluxes(P, T, 2));
+            prof_mark(P, 3);
+            HIPCHK(launch_zupdate(L, P.prec, P.zshape, T, P.d_zt, P.nzt, s, 0));
+            prof_mark(P, 9);
+            if (child) HIPCHK(launch_fr_coarse(L, P.prec, m->d_all, m->d_f + l, m->hf[l].froff[6] / 5, RK_W[s - 1]));
+            if (l > 0) HIPCHK(launch_fr_fine(L, P.prec, m->d_all, m->d_f + (l - 1), m->hf[l - 1].froff[6] / 5, RK_W[s - 1]));
+        }
+        CHK(plan_ghost_combine(P, 3, m->d_fst[3 * l + 2]));
+    } else {
     if (l > 0) HIPCHK(launch_copy_state(L, h, h.U, h.Un));
     if (l > 0) CHK(fmr_interp(m, l));
     for (int s = 1; s <= 3; ++s) {
@@ -1701,6 +1785,7 @@ static int fmr_advance(bsn_fmr* m, int l) {
         CHK(plan_update(P, s, 0));
     }
     CHK(plan_ghost_combine(P, 3));
+    }
     if (child) {
         const FmrDev& f = m->hf[l];
         HIPCHK(launch_child_dt(L, m->d_all, l, l + 1, f.ratio));
@@ -1713,6 +1798,13 @@ static int fmr_advance(bsn_fmr* m, int l) {
         HIPCHK(launch_bc_correct(L, m->d_all, m->d_f + l, f.froff[6] / 5));
         CHK(plan_floors(P, 0));
     }
+    return BSN_OK;
+}
+
+extern "C" int bsn_fmr_set_mode(bsn_fmr* m, int inplace) {
+    if (!m || (inplace != 0 && inplace != 1)) return set_err(BSN_ERR_ARG, "bad arguments");
+    m->inplace = inplace;
+    graph_drop(m->gexec, m->graph);
     return BSN_OK;
 }
 

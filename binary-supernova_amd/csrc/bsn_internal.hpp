@@ -72,6 +72,8 @@ struct GridDev {
     double idx, idy, idz;
     double gamma, gfac, rho_floor, P_floor;
     float gf32[3];     // gamma, gfac and 1/(gamma - 1) rounded to FP32 (split flux)
+    int nzkeep;        // z interfaces whose fluxes the fused z-sweep + update kernel
+    int zkeep[4];      // stores to F[2] (the FMR flux registers read them)
 };
 
 // A tile of work: which grid, and three integer coordinates whose meaning is

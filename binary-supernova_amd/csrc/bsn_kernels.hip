@@ -190,8 +190,11 @@ __device__ __forceinline__ void load_cell(const GridDev& G, int i, int j, int k,
     const int64_t o = uidx(G, si, sj, sk);
 #pragma unroll
     for (int q = 0; q < 5; ++q) u[q] = gmem(G.U)[q * G.fs + o];
-    if (FLOOR && (G.lmode & 2) && si == i && sj == j && sk == k)
-        apply_floor<FAST>(u, G.rho_floor, G.P_floor, G.gamma);
+    // only active cells are floored (callers pass active cross indices; a
+    // ghost cell holds the raw value of a ghost fill or a prolongation)
+    const bool active = AX == 0 ? (i >= NG && i < NG + G.nx)
+                      : AX == 1 ? (j >= NG && j < NG + G.ny) : (k >= NG && k < NG + G.nz);
+    if (FLOOR && (G.lmode & 2) && active) apply_floor<FAST>(u, G.rho_floor, G.P_floor, G.gamma);
 }
 
 // Floors on active cells (apply_floors_3d!, euler3d.jl:73-80); with copy_un
@@ -215,7 +218,10 @@ __global__ void k_floors(const GridDev* __restrict__ grids, const int64_t* __res
 #pragma unroll
             for (int q = 0; q < 5; ++q) G.Un[q * G.fs + c] = u[q];
         }
-        if (apply_floor(u, G.rho_floor, G.P_floor, G.gamma)) {
+        // the split form floors with the same division wherever it floors
+        const bool fired = G.prec == 1 ? apply_floor<true>(u, G.rho_floor, G.P_floor, G.gamma)
+                                       : apply_floor<false>(u, G.rho_floor, G.P_floor, G.gamma);
+        if (fired) {
 #pragma unroll
             for (int q = 0; q < 5; ++q) G.U[q * G.fs + c] = u[q];
         }
@@ -678,6 +684,19 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
                 if (PF == 1 && f > k0) operands(k);
                 T F[5];
                 Prec<T>::template flux<2>(c, p, G, F);
+                if (G.nzkeep > 0) {
+                    // an interface the flux registers read: store it as the
+                    // z-flux sweep would (adjacent tiles both compute their
+                    // shared interface and store the same values)
+                    bool keep = false;
+                    for (int n = 0; n < G.nzkeep; ++n) keep |= G.zkeep[n] == f;
+                    if (keep) {
+                        const auto Fz = gmem((T*)G.F[2]);
+                        const int64_t oz = ((int64_t)f * G.ny + j) * G.fp[2] + i;
+#pragma unroll
+                        for (int q = 0; q < 5; ++q) Fz[q * G.ffs[2] + oz] = F[q];
+                    }
+                }
                 if (PF == 0 && f < kend) fetch(f + 6);
                 if (PF != 1 && f > k0) operands(k);
                 if (f > k0) {
@@ -967,8 +986,10 @@ __device__ __forceinline__ void coarse_stencil(int ig_f1, int c_lo, int ratio, i
 // Uc(alpha) = (1 - alpha) Ucn + alpha Ucnp1 (fmr3d.jl:551), evaluated per
 // coarse value on the fly.  all_cells = 1 fills every fine cell
 // (_prolong_all!), else only ghost cells in the reference's slab partition.
+// The values go to each of the dst.n buffers in the child's layout, or to
+// the child's state when dst.n is 0.
 __global__ void k_prolong(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr,
-                          const double* __restrict__ Uca, int all_cells) {
+                          const double* __restrict__ Uca, int all_cells, ProlongDst dst) {
     const FmrDev& M = *fmr;
     const GridDev& C = grids[M.parent];
     const GridDev& Fg = grids[M.child];
@@ -1009,8 +1030,14 @@ __global__ void k_prolong(const GridDev* __restrict__ grids, const FmrDev* __res
                 }
             }
         }
+        if (dst.n == 0) {
 #pragma unroll
-        for (int q = 0; q < 5; ++q) Fg.U[q * Fg.fs + d] = val[q];
+            for (int q = 0; q < 5; ++q) Fg.U[q * Fg.fs + d] = val[q];
+        }
+        for (int b = 0; b < dst.n; ++b) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) dst.p[b][q * Fg.fs + d] = val[q];
+        }
     }
 }
 
@@ -1462,10 +1489,10 @@ hipError_t launch_totals(const LaunchCtx& L, const GridDev& G, double* part, int
 }
 
 hipError_t launch_prolong(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total,
-                          const double* Uca, int all_cells) {
+                          const double* Uca, int all_cells, ProlongDst dst) {
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_prolong, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids, fmr,
-                       Uca, all_cells);
+                       Uca, all_cells, dst);
     return hipGetLastError();
 }
 

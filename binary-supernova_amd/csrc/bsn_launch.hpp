@@ -45,8 +45,13 @@ hipError_t launch_combine_all(const LaunchCtx& L, const GridDev& G, int stage, c
 hipError_t launch_copy_state(const LaunchCtx& L, const GridDev& G, const double* src, double* dst);
 hipError_t launch_totals(const LaunchCtx& L, const GridDev& G, double* part, int nblocks, double dV,
                          double* out);
+// Buffers (child layout) a prolongation writes; none = the child's state.
+struct ProlongDst {
+    int n;
+    double* p[3];
+};
 hipError_t launch_prolong(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total,
-                          const double* Uca, int all_cells);
+                          const double* Uca, int all_cells, ProlongDst dst = ProlongDst{0, {}});
 hipError_t launch_time_interp(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr,
                               int64_t total, double alpha);
 hipError_t launch_restrict(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total);

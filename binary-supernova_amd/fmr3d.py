@@ -51,6 +51,12 @@ class FMRHierarchy:
         except Exception:
             pass
 
+    def set_mode(self, inplace: bool) -> None:
+        """Level step form: False (default) fuses the z sweep with each
+        stage update; True steps every level in place with all three flux
+        arrays materialised.  Both give the same bits."""
+        call("bsn_fmr_set_mode", self.handle, int(bool(inplace)))
+
     def prolong_all(self, level: int) -> None:
         """_prolong_all! (src/fmr3d.jl:206-214)"""
         call("bsn_fmr_prolong_all", self.handle, level)

@@ -251,6 +251,12 @@ int bsn_batch_times(bsn_batch* b, double* t, double* dt);
 int bsn_fmr_create(bsn_ctx* ctx, int nlevels, bsn_grid* const* levels, const int64_t* lo_hi,
                    const int* ratio, bsn_fmr** out);
 int bsn_fmr_destroy(bsn_fmr* m);
+/* Level step form: 0 (default) = per stage the x and y sweeps and the z sweep
+ * fused with the stage update, which also stores the z faces the flux
+ * registers read; 1 = in place with all three flux arrays materialised (the
+ * order of src/fmr3d.jl:497-530).  Both give the same bits.  The fused form
+ * holds two more state buffers per level. */
+int bsn_fmr_set_mode(bsn_fmr* m, int inplace);
 /* _prolong_all! / _prolong_fine_ghosts! of level l >= 1 from its parent's
  * current state (src/fmr3d.jl:206-255). */
 int bsn_fmr_prolong_all(bsn_fmr* m, int level);

@@ -167,12 +167,19 @@ def _fmr_case(r):
     return Uc, Uf, dxc, lo_hi, gm
 
 
+# every level step in both forms: the z sweep fused with the stage update
+# (default) and in place with all three flux arrays materialised
+FMR_FORMS = ["fused", "inplace"]
+
+
+@pytest.mark.parametrize("form", FMR_FORMS)
 @pytest.mark.parametrize("ratio", [4, 2])
-def test_fmr_two_level_bitwise(ratio):
-    print(f"fmr r={ratio}", flush=True)
+def test_fmr_two_level_bitwise(ratio, form):
+    print(f"fmr r={ratio} {form}", flush=True)
     Uc, Uf, dxc, lo_hi, gm = _fmr_case(ratio)
     G = bsn.FMRGrid3D(Uc, 8, 8, 8, dxc, dxc, dxc, *lo_hi, ratio, gm, bc="outflow",
                       rho_floor=1e-10, P_floor=1e-5)
+    G.set_mode(form == "inplace")
     Ufo = np.zeros_like(Uf)
     O.prolong_all(Ufo, Uc, lo_hi, ratio)
     np.testing.assert_array_equal(G.fine.get_state(), Ufo)       # _prolong_all!
@@ -189,12 +196,14 @@ def test_fmr_two_level_bitwise(ratio):
         np.testing.assert_array_equal(G.fine.get_state(), Uf)
 
 
-def test_fmr_two_level_with_phi_bitwise():
-    print("fmr phi", flush=True)
+@pytest.mark.parametrize("form", FMR_FORMS)
+def test_fmr_two_level_with_phi_bitwise(form):
+    print(f"fmr phi {form}", flush=True)
     Uc, Uf, dxc, lo_hi, gm = _fmr_case(2)
     rng = np.random.default_rng(4)
     phc, phf = rng.normal(size=(8, 8, 8)), rng.normal(size=(8, 8, 8))
     G = bsn.FMRGrid3D(Uc, 8, 8, 8, dxc, dxc, dxc, *lo_hi, 2, gm, rho_floor=1e-10, P_floor=1e-5)
+    G.set_mode(form == "inplace")
     G.fine.set_state(Uf)
     G.prolong_fine_ghosts(1)
     O.prolong_fine_ghosts(Uf, Uc, lo_hi, 2)
@@ -207,8 +216,9 @@ def test_fmr_two_level_with_phi_bitwise():
     np.testing.assert_array_equal(G.fine.get_state(), Uf)
 
 
-def test_fmr_three_level_bitwise():
-    print("fmr 3 levels", flush=True)
+@pytest.mark.parametrize("form", FMR_FORMS)
+def test_fmr_three_level_bitwise(form):
+    print(f"fmr 3 levels {form}", flush=True)
     gm = 5 / 3
     n0, dx0 = 16, 1.0 / 16
     U0 = P.sedov_ic_3d(n0, n0, n0, dx0, dx0, dx0, gm, E_blast=0.5, r_inject=0.2, rho_bg=1.0,
@@ -219,6 +229,7 @@ def test_fmr_three_level_bitwise():
     d2 = (16, 20, 8)
     H = bsn.FMRHierarchy([(n0,) * 3, d1, d2], [(dx0,) * 3, (dx0 / 2,) * 3, (dx0 / 4,) * 3],
                          [b1, b2], [2, 2], gm, "outflow", 1e-10, 1e-5)
+    H.set_mode(form == "inplace")
     H.levels[0].set_state(U0)
     H.prolong_all(1)
     H.prolong_all(2)
@@ -235,6 +246,39 @@ def test_fmr_three_level_bitwise():
         H.step(dt)
     for lev, U in zip(H.levels, (U0, U1, U2)):
         np.testing.assert_array_equal(lev.get_state(), U)
+
+
+@pytest.mark.parametrize("prec", ["split", "fp64"])
+def test_fmr_forms_and_graph_agree(prec):
+    # bsn_fmr_run (device dt, whole step captured in a hipGraph) in the fused
+    # form against the in-place form stepped without a graph: three levels
+    # with a phi on every level, same bits in both precisions
+    print(f"fmr forms {prec}", flush=True)
+    gm = 5 / 3
+    n0, dx0 = 16, 1.0 / 16
+    U0 = P.sedov_ic_3d(n0, n0, n0, dx0, dx0, dx0, gm, E_blast=0.5, r_inject=0.2, rho_bg=1.0,
+                       P_floor=1e-4, x_offset=-0.5, y_offset=-0.5, z_offset=-0.5)
+    O.fill_ghost(U0, "outflow")
+    b1, b2 = (5, 12, 5, 12, 6, 11), (5, 12, 4, 13, 5, 8)
+    dims = [(n0,) * 3, (16, 16, 12), (16, 20, 8)]
+    dxs = [(dx0,) * 3, (dx0 / 2,) * 3, (dx0 / 4,) * 3]
+    rng = np.random.default_rng(12)
+    phis = [0.05 * rng.normal(size=(d[2], d[1], d[0])) for d in dims]
+    out = {}
+    for form in ("fused", "inplace"):
+        H = bsn.FMRHierarchy(dims, dxs, [b1, b2], [2, 2], gm, "outflow", 1e-10, 1e-5,
+                             precision=prec)
+        H.set_mode(form == "inplace")
+        H.levels[0].set_state(U0)
+        H.prolong_all(1)
+        H.prolong_all(2)
+        H.set_phis(phis)
+        H.run(0.4, 4, graph=form == "fused")
+        out[form] = [lev.get_state() for lev in H.levels] + [np.array(H.time())]
+        del H
+    for a, b in zip(out["fused"], out["inplace"]):
+        np.testing.assert_array_equal(a, b)
+    assert out["fused"][-1][0] > 0
 
 
 def test_batch_matches_single_bitwise():
