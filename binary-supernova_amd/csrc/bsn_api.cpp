@@ -354,6 +354,12 @@ static int plan_new(bsn_ctx* ctx, const std::vector<bsn_grid*>& grids, Plan** ou
         if (g->h.prec != P->prec) { delete P; return set_err(BSN_ERR_ARG, "grids of mixed precision"); }
         if (g->ctx != ctx) { delete P; return set_err(BSN_ERR_ARG, "grids of different contexts"); }
     }
+    // starting shapes (kept until bsn_batch_autotune or set_*shape*): the
+    // ones the autotuner keeps for the 256^3 box on MI355X.  Split: x 64x4,
+    // y and z 32x8, the fused kernel with the LDS ring, the early load and
+    // balanced runs; FP64: x 16x16, y and z 32x8, balanced runs.
+    if (P->prec == 1) { P->shape[0] = 0; P->shape[1] = 2; P->shape[2] = 2; P->zshape = 11; }
+    else              { P->shape[0] = 2; P->shape[1] = 2; P->shape[2] = 2; P->zshape = 2; }
     int r = plan_build(*P);
     if (r != BSN_OK) { plan_free(P); return r; }
     *out = P;

@@ -219,7 +219,10 @@ int bsn_cfl_dt_3d_host(const double* U, int64_t nx, int64_t ny, int64_t nz, doub
 int bsn_batch_create(bsn_ctx* ctx, int ngrids, bsn_grid* const* grids, bsn_batch** out);
 int bsn_batch_destroy(bsn_batch* b);
 /* Times every tile shape of every kernel on the current states and keeps the
- * fastest; writes a JSON report of the choice into report (may be NULL). */
+ * fastest; writes a JSON report of the choice into report (may be NULL).
+ * Without it a batch (and every FMR level and single-grid call) starts from
+ * the shapes the autotuner keeps for a 256^3 box: split x 0, y/z 2, fused
+ * kernel 11; FP64 x 2, y/z 2, fused kernel 2. */
 int bsn_batch_autotune(bsn_batch* b, char* report, int report_len);
 /* Sets the flux tile shape ids explicitly (x: 0..2, y/z: 0..3). */
 int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz);
