@@ -451,6 +451,7 @@ def bench_fmr(bsn, args):
         log(f"fmr 3 levels: {prec}")
         H = hierarchy(prec)
         init = [g.get_state() for g in H.levels]
+        tune = H.autotune()
         H.run(0.4, args.warmup, graph=True)
         for g, U in zip(H.levels, init):
             g.set_state(U)
@@ -459,7 +460,8 @@ def bench_fmr(bsn, args):
         H.run(0.4, steps, graph=True)
         wall = time.perf_counter() - t0
         res[prec] = dict(wall=wall, zones=H.zone_updates_per_step() * steps,
-                         states=[g.get_state() for g in H.levels], init=init, t=H.time()[0])
+                         states=[g.get_state() for g in H.levels], init=init, t=H.time()[0],
+                         tune=[r["zupdate"]["best"] for r in tune])
         del H
     v = {pr: res[pr]["zones"] / res[pr]["wall"] for pr in res}
     A, B = res["fp64"]["states"][0], res["split"]["states"][0]
@@ -490,6 +492,7 @@ def bench_fmr(bsn, args):
                 l1_density_split_vs_fp64_base=l1,
                 l1_note=f"root density after {steps} steps of one dt sequence (the FP64 form's CFL dt)",
                 device_dt_end_times=dict(fp64=res["fp64"]["t"], split=res["split"]["t"]),
+                fused_kernel_shapes=dict(fp64=res["fp64"]["tune"], split=res["split"]["tune"]),
                 mass_drift_fp64_base=float(abs(tA[0] - t0[0]) / t0[0]),
                 mass_drift_split_base=float(abs(tB[0] - t0[0]) / t0[0]),
                 energy_drift_fp64_base=float(abs(tA[4] - t0[4]) / t0[4]),

@@ -1357,9 +1357,9 @@ extern "C" int bsn_batch_set_mode(bsn_batch* b, int inplace) {
     return BSN_OK;
 }
 
-extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
-    if (!b) return set_err(BSN_ERR_ARG, "NULL batch");
-    Plan& P = *b->plan;
+// Times every tile shape of every kernel of a plan on its current states and
+// keeps the fastest; appends a JSON object of the timings to rep.
+static int plan_autotune(Plan& P, std::string& out) {
     HIPCHK(hipSetDevice(P.ctx->device));
     CHK(plan_refresh(P));
     const LaunchCtx L = P.ctx->L();
@@ -1435,6 +1435,14 @@ extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
     HIPCHK(hipStreamSynchronize(L.stream));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    out += rep;
+    return BSN_OK;
+}
+
+extern "C" int bsn_batch_autotune(bsn_batch* b, char* report, int report_len) {
+    if (!b) return set_err(BSN_ERR_ARG, "NULL batch");
+    std::string rep;
+    CHK(plan_autotune(*b->plan, rep));
     if (report && report_len > 0) snprintf(report, (size_t)report_len, "%s", rep.c_str());
     return BSN_OK;
 }
@@ -1811,6 +1819,24 @@ extern "C" int bsn_fmr_set_mode(bsn_fmr* m, int inplace) {
     if (!m || (inplace != 0 && inplace != 1)) return set_err(BSN_ERR_ARG, "bad arguments");
     m->inplace = inplace;
     graph_drop(m->gexec, m->graph);
+    return BSN_OK;
+}
+
+// Each level's plan tuned on the level's current state (the timing launches
+// write the level's flux arrays and its Un buffer, which every level step
+// rebuilds before reading).
+extern "C" int bsn_fmr_autotune(bsn_fmr* m, char* report, int report_len) {
+    if (!m) return set_err(BSN_ERR_ARG, "NULL fmr");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    CHK(fmr_refresh(m));
+    std::string rep = "[";
+    for (int l = 0; l < m->nlev; ++l) {
+        if (l) rep += ", ";
+        CHK(plan_autotune(*m->plans[l], rep));
+    }
+    rep += "]";
+    graph_drop(m->gexec, m->graph);
+    if (report && report_len > 0) snprintf(report, (size_t)report_len, "%s", rep.c_str());
     return BSN_OK;
 }
 

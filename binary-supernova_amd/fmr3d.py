@@ -6,6 +6,7 @@ recursively, each advanced by the two-level algorithm of fmr3d_step!).
 from __future__ import annotations
 
 import ctypes as C
+import json
 
 import numpy as np
 
@@ -56,6 +57,13 @@ class FMRHierarchy:
         stage update; True steps every level in place with all three flux
         arrays materialised.  Both give the same bits."""
         call("bsn_fmr_set_mode", self.handle, int(bool(inplace)))
+
+    def autotune(self) -> list:
+        """Tile and kernel shapes of every level timed on its current state
+        (the fastest kept); returns the per-level timing reports."""
+        buf = C.create_string_buffer(16384)
+        call("bsn_fmr_autotune", self.handle, buf, 16384)
+        return json.loads(buf.value.decode())
 
     def prolong_all(self, level: int) -> None:
         """_prolong_all! (src/fmr3d.jl:206-214)"""

@@ -260,6 +260,9 @@ int bsn_fmr_destroy(bsn_fmr* m);
  * order of src/fmr3d.jl:497-530).  Both give the same bits.  The fused form
  * holds two more state buffers per level. */
 int bsn_fmr_set_mode(bsn_fmr* m, int inplace);
+/* bsn_batch_autotune for every level on its current state; report gets a
+ * JSON array of the per-level reports.  The level states are not changed. */
+int bsn_fmr_autotune(bsn_fmr* m, char* report, int report_len);
 /* _prolong_all! / _prolong_fine_ghosts! of level l >= 1 from its parent's
  * current state (src/fmr3d.jl:206-255). */
 int bsn_fmr_prolong_all(bsn_fmr* m, int level);

@@ -273,6 +273,9 @@ def test_fmr_forms_and_graph_agree(prec):
         H.prolong_all(1)
         H.prolong_all(2)
         H.set_phis(phis)
+        if form == "fused":
+            rep = H.autotune()              # leaves the level states as they were
+            assert len(rep) == 3 and all("zupdate" in r for r in rep)
         H.run(0.4, 4, graph=form == "fused")
         out[form] = [lev.get_state() for lev in H.levels] + [np.array(H.time())]
         del H
