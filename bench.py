@@ -126,6 +126,19 @@ def totals(U, dV):
     return np.array([a[..., q].sum(dtype=np.float64) for q in range(5)]) * dV
 
 
+def host_cpu() -> dict:
+    """The host the CPU baseline ran on: CPU model and logical CPUs."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(model=model, nproc=os.cpu_count() or 1)
+
+
 def cpu_baseline(U, dims, d, gamma, bc, floors, nsteps=1):
     sys.path.insert(0, str(ROOT))
     from oracle import oracle as O
@@ -138,7 +151,7 @@ def cpu_baseline(U, dims, d, gamma, bc, floors, nsteps=1):
     nx, ny, nz = dims
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=nx * ny * nz * nsteps / sec, unit="zone-updates/s", cores=cores,
-                kind="port",
+                kind="port", label="reference-restatement CPU baseline", host=host_cpu(),
                 sample=f"{nsteps} RK3 step(s) (cfl_dt_3d + euler3d_step) of the same "
                        f"{nx}x{ny}x{nz} state, FP64 C/OpenMP restatement, {sec:.1f} s")
 
@@ -571,6 +584,7 @@ def bench_selfgrav(bsn, args):
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = dict(
             value=cells / sec, unit="zone-updates/s", cores=cores, kind="port",
+            label="reference-restatement CPU baseline", host=host_cpu(),
             sample=f"1 RK3 step with three numpy FFT Poisson solves of the same {n}^3 state, "
                    f"FP64 C/OpenMP restatement, {sec:.1f} s")
     return out
@@ -664,7 +678,7 @@ def bench_binary(bsn, args):
         sec = time.perf_counter() - t0
         out["cpu_baseline"] = dict(
             value=1e3 * sec, unit="ms per step (gas forces + accretion sums, 2 bodies)", cores=1,
-            kind="port", sample=f"gas_force_on_bh and accrete! sums of both bodies on the same "
+            kind="port", label="reference-restatement CPU baseline", host=host_cpu(), sample=f"gas_force_on_bh and accrete! sums of both bodies on the same "
                                 f"{n}^3 state, sequential FP64 C restatement, {sec:.2f} s")
         out["gpu_ms_force_and_accrete"] = out["ms_per_call"]["force"] + out["ms_per_call"]["accrete"]
     return out
