@@ -1256,7 +1256,8 @@ static hipError_t launch_flux_t(const LaunchCtx& L, const GridDev* grids, const 
 template <int DIR, typename T>
 static hipError_t launch_flux_shape(const LaunchCtx& L, int shape, const GridDev* grids,
                                     const Tile* tiles, int ntiles, int max_wg) {
-    // shape ids: x sweep 0: 64x4, 1: 32x8, 2: 16x16;  y/z sweep 0: 64x4, 1: 64x8, 2: 32x8, 3: 64x2
+    // shape ids: x sweep 0: 64x4, 1: 32x8, 2: 16x16;  y/z sweep 0: 64x4, 1: 64x8, 2: 32x8, 3: 64x2,
+    // 4: 32x16
     if (DIR == 0) {
         switch (shape) {
             case 1: return launch_flux_t<DIR, T, 32, 8>(L, grids, tiles, ntiles, max_wg);
@@ -1268,6 +1269,7 @@ static hipError_t launch_flux_shape(const LaunchCtx& L, int shape, const GridDev
             case 1: return launch_flux_t<DIR, T, 64, 8>(L, grids, tiles, ntiles, max_wg);
             case 2: return launch_flux_t<DIR, T, 32, 8>(L, grids, tiles, ntiles, max_wg);
             case 3: return launch_flux_t<DIR, T, 64, 2>(L, grids, tiles, ntiles, max_wg);
+            case 4: return launch_flux_t<DIR, T, 32, 16>(L, grids, tiles, ntiles, max_wg);
             default: return launch_flux_t<DIR, T, 64, 4>(L, grids, tiles, ntiles, max_wg);
         }
     }
@@ -1275,12 +1277,12 @@ static hipError_t launch_flux_shape(const LaunchCtx& L, int shape, const GridDev
 
 void flux_shape_dims(int dir, int shape, int& W, int& H) {
     static const int xs[3][2] = {{64, 4}, {32, 8}, {16, 16}};
-    static const int ys[4][2] = {{64, 4}, {64, 8}, {32, 8}, {64, 2}};
+    static const int ys[5][2] = {{64, 4}, {64, 8}, {32, 8}, {64, 2}, {32, 16}};
     if (dir == 0) { if (shape < 0 || shape > 2) shape = 0; W = xs[shape][0]; H = xs[shape][1]; }
-    else          { if (shape < 0 || shape > 3) shape = 0; W = ys[shape][0]; H = ys[shape][1]; }
+    else          { if (shape < 0 || shape > 4) shape = 0; W = ys[shape][0]; H = ys[shape][1]; }
 }
 
-int flux_shape_count(int dir) { return dir == 0 ? 3 : 4; }
+int flux_shape_count(int dir) { return dir == 0 ? 3 : 5; }
 
 hipError_t launch_flux(const LaunchCtx& L, int dir, int prec, int shape, const GridDev* grids,
                        const Tile* tiles, int ntiles, int max_wg) {

@@ -224,7 +224,7 @@ int bsn_batch_destroy(bsn_batch* b);
  * the shapes the autotuner keeps for a 256^3 box: split x 0, y/z 2, fused
  * kernel 11; FP64 x 2, y/z 2, fused kernel 2. */
 int bsn_batch_autotune(bsn_batch* b, char* report, int report_len);
-/* Sets the flux tile shape ids explicitly (x: 0..2, y/z: 0..3). */
+/* Sets the flux tile shape ids explicitly (x: 0..2, y/z: 0..4). */
 int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz);
 /* Sets the shape id (0..11) of the fused z-sweep + stage-update kernel used
  * by the out-of-place step: shape = 3 * variant + segmentation.  Variant 0

@@ -332,31 +332,35 @@ def test_device_dt_run_graph_equals_host_loop():
         np.testing.assert_array_equal(A, V)
 
 
-def test_every_tile_shape_gives_the_same_bits():
-    print("shapes", flush=True)
+@pytest.mark.parametrize("prec", ["fp64", "split"])
+def test_every_tile_shape_gives_the_same_bits(prec):
+    print(f"shapes {prec}", flush=True)
     s = (40, 36, 20)
     U = random_state(*s, seed=21)
     ref = None
+    # x shapes 0..2, y/z shapes 0..4; the z shape acts in the in-place form
     for sx in range(3):
-        for sy in range(4):
-            g = bsn.Grid(*s).set_state(U)
-            g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
-            b = bsn.Batch([g])
-            b.set_shapes(sx, sy, sy)
-            b.step([1e-3])
-            out = g.get_state()
-            if ref is None:
-                ref = out
-            np.testing.assert_array_equal(out, ref)
+        for sy in range(5):
+            for inplace in (False, True):
+                g = bsn.Grid(*s, prec).set_state(U)
+                g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
+                b = bsn.Batch([g])
+                b.set_mode(inplace)
+                b.set_shapes(sx, sy, sy)
+                b.step([1e-3])
+                out = g.get_state()
+                if ref is None:
+                    ref = out
+                np.testing.assert_array_equal(out, ref)
     for sz in range(12):
-        g = bsn.Grid(*s).set_state(U)
+        g = bsn.Grid(*s, prec).set_state(U)
         g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
         b = bsn.Batch([g])
         b.set_update_shape(sz)
         b.step([1e-3])
         np.testing.assert_array_equal(g.get_state(), ref)
     b.autotune()
-    g = bsn.Grid(*s).set_state(U)
+    g = bsn.Grid(*s, prec).set_state(U)
     g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
     b = bsn.Batch([g])
     rep = b.autotune()
