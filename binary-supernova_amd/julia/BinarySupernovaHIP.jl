@@ -17,7 +17,8 @@ module BinarySupernovaHIP
 
 export HIPGrid, euler3d_step!, euler3d_rhs!, cfl_dt_3d, apply_floors_3d!,
        fill_ghost_3d_outflow!, fill_ghost_3d_periodic!, rk3_step!, HIPFMR,
-       fmr3d_step!, cfl_dt_fmr3d, set_phi!, download, add_bh_gravity_source!,
+       fmr3d_step!, cfl_dt_fmr3d, fmr_run!, fmr_autotune!, fmr_set_mode!, set_phi!, download,
+       add_bh_gravity_source!,
        add_sink_sources!, set_point_masses!, set_sinks!, solve_poisson_isolated,
        set_self_gravity!, gas_force_on_bh, accrete!, gas_diagnostics, gas_energy_total,
        gas_kinetic_total, gas_momentum_total, gas_angular_momentum_total, bound_gas_mass,
@@ -215,6 +216,28 @@ function cfl_dt_fmr3d(G::HIPFMR, cfl::Real)
     check(ccall((:bsn_cfl_dt_fmr3d, libbsn), Cint, (Ptr{Cvoid}, Float64, Ref{Float64}),
                 G.handle, cfl, dt))
     return dt[]
+end
+
+"""fmr_run!(G, cfl, nsteps; t_end = 0, graph = true) -- nsteps CFL-limited root
+steps with dt computed on the device (one hipGraph per step when graph)."""
+function fmr_run!(G::HIPFMR, cfl::Real, nsteps::Int; t_end::Real = 0.0, graph::Bool = true)
+    check(ccall((:bsn_fmr_run, libbsn), Cint, (Ptr{Cvoid}, Float64, Float64, Cint, Cint),
+                G.handle, cfl, t_end, nsteps, graph ? 1 : 0))
+end
+
+"""fmr_autotune!(G) -- kernel shapes of every level timed on its current state;
+returns the JSON report."""
+function fmr_autotune!(G::HIPFMR)
+    buf = zeros(UInt8, 16384)
+    check(ccall((:bsn_fmr_autotune, libbsn), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Cint),
+                G.handle, buf, length(buf)))
+    return unsafe_string(pointer(buf))
+end
+
+"""fmr_set_mode!(G, inplace) -- level step form: false fuses the z sweep with the
+stage update (the default), true steps in place; both give the same bits."""
+function fmr_set_mode!(G::HIPFMR, inplace::Bool)
+    check(ccall((:bsn_fmr_set_mode, libbsn), Cint, (Ptr{Cvoid}, Cint), G.handle, inplace ? 1 : 0))
 end
 
 # ---------------------------------------------------------------------------
