@@ -1124,6 +1124,7 @@ __device__ __forceinline__ double flux_at(const GridDev& G, int dir, int q, int 
 template <typename T>
 __global__ void k_fr_coarse(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr,
                             double weight) {
+    const int q = blockIdx.y;                   // one field per grid row
     const FmrDev& M = *fmr;
     const GridDev& C = grids[M.parent];
     const double fac = weight * C.scal[SCAL_DT];
@@ -1139,14 +1140,12 @@ __global__ void k_fr_coarse(const GridDev* __restrict__ grids, const FmrDev* __r
         const int hi = face & 1;
         // reference interface NG + ci_lo - 1 (lo) or NG + ci_hi (hi) is
         // device interface f = ci_lo - 1 or ci_hi (active-cell faces)
-        for (int q = 0; q < 5; ++q) {
-            double F;
-            if (face < 2)      F = flux_at<T>(C, 0, q, hi ? M.hi[0] : M.lo[0] - 1, M.lo[1] - 1 + a, M.lo[2] - 1 + b);
-            else if (face < 4) F = flux_at<T>(C, 1, q, hi ? M.hi[1] : M.lo[1] - 1, M.lo[0] - 1 + a, M.lo[2] - 1 + b);
-            else               F = flux_at<T>(C, 2, q, hi ? M.hi[2] : M.lo[2] - 1, M.lo[0] - 1 + a, M.lo[1] - 1 + b);
-            double* R = &M.frc[M.froff[face] + q + 5 * loc];
-            *R = *R + fac * F;
-        }
+        double F;
+        if (face < 2)      F = flux_at<T>(C, 0, q, hi ? M.hi[0] : M.lo[0] - 1, M.lo[1] - 1 + a, M.lo[2] - 1 + b);
+        else if (face < 4) F = flux_at<T>(C, 1, q, hi ? M.hi[1] : M.lo[1] - 1, M.lo[0] - 1 + a, M.lo[2] - 1 + b);
+        else               F = flux_at<T>(C, 2, q, hi ? M.hi[2] : M.lo[2] - 1, M.lo[0] - 1 + a, M.lo[1] - 1 + b);
+        double* R = &M.frc[M.froff[face] + q + 5 * loc];
+        *R = *R + fac * F;
     }
 }
 
@@ -1155,6 +1154,7 @@ __global__ void k_fr_coarse(const GridDev* __restrict__ grids, const FmrDev* __r
 template <typename T>
 __global__ void k_fr_fine(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr,
                           double weight) {
+    const int q = blockIdx.y;                   // one field per grid row
     const FmrDev& M = *fmr;
     const GridDev& Fg = grids[M.child];
     const int r = M.ratio;
@@ -1169,22 +1169,20 @@ __global__ void k_fr_fine(const GridDev* __restrict__ grids, const FmrDev* __res
         face_dims(M, face, n1, n2);
         const int a = (int)(loc % n1), b = (int)(loc / n1);
         const int hi = face & 1;
-        for (int q = 0; q < 5; ++q) {
-            double* R = &M.frf[M.froff[face] + q + 5 * loc];
-            double acc = *R;
-            // outer loop over the second register index (kf or jf), inner
-            // over the first (jf or ia), as the reference
-            for (int ob = 0; ob < r; ++ob)
-                for (int oa = 0; oa < r; ++oa) {
-                    const int fa = a * r + oa, fb = b * r + ob;   // 0-based fine active
-                    double F;
-                    if (face < 2)      F = flux_at<T>(Fg, 0, q, hi ? Fg.nx : 0, fa, fb);
-                    else if (face < 4) F = flux_at<T>(Fg, 1, q, hi ? Fg.ny : 0, fa, fb);
-                    else               F = flux_at<T>(Fg, 2, q, hi ? Fg.nz : 0, fa, fb);
-                    acc = acc + fac * F;
-                }
-            *R = acc;
-        }
+        double* R = &M.frf[M.froff[face] + q + 5 * loc];
+        double acc = *R;
+        // outer loop over the second register index (kf or jf), inner over
+        // the first (jf or ia), as the reference
+        for (int ob = 0; ob < r; ++ob)
+            for (int oa = 0; oa < r; ++oa) {
+                const int fa = a * r + oa, fb = b * r + ob;   // 0-based fine active
+                double F;
+                if (face < 2)      F = flux_at<T>(Fg, 0, q, hi ? Fg.nx : 0, fa, fb);
+                else if (face < 4) F = flux_at<T>(Fg, 1, q, hi ? Fg.ny : 0, fa, fb);
+                else               F = flux_at<T>(Fg, 2, q, hi ? Fg.nz : 0, fa, fb);
+                acc = acc + fac * F;
+            }
+        *R = acc;
     }
 }
 
@@ -1518,19 +1516,21 @@ hipError_t launch_fr_zero(const LaunchCtx& L, const FmrDev* fmr, int64_t total) 
 
 hipError_t launch_fr_coarse(const LaunchCtx& L, int prec, const GridDev* grids, const FmrDev* fmr,
                             int64_t nelem, double weight) {
+    const dim3 grid(flat_blocks(nelem, L.cus), 5);     // one grid row per field
     if (prec == 0)
-        hipLaunchKernelGGL(k_fr_coarse<double>, dim3(flat_blocks(nelem, L.cus)), dim3(256), 0, L.stream, grids, fmr, weight);
+        hipLaunchKernelGGL(k_fr_coarse<double>, grid, dim3(256), 0, L.stream, grids, fmr, weight);
     else
-        hipLaunchKernelGGL(k_fr_coarse<float>, dim3(flat_blocks(nelem, L.cus)), dim3(256), 0, L.stream, grids, fmr, weight);
+        hipLaunchKernelGGL(k_fr_coarse<float>, grid, dim3(256), 0, L.stream, grids, fmr, weight);
     return hipGetLastError();
 }
 
 hipError_t launch_fr_fine(const LaunchCtx& L, int prec, const GridDev* grids, const FmrDev* fmr,
                           int64_t nelem, double weight) {
+    const dim3 grid(flat_blocks(nelem, L.cus), 5);     // one grid row per field
     if (prec == 0)
-        hipLaunchKernelGGL(k_fr_fine<double>, dim3(flat_blocks(nelem, L.cus)), dim3(256), 0, L.stream, grids, fmr, weight);
+        hipLaunchKernelGGL(k_fr_fine<double>, grid, dim3(256), 0, L.stream, grids, fmr, weight);
     else
-        hipLaunchKernelGGL(k_fr_fine<float>, dim3(flat_blocks(nelem, L.cus)), dim3(256), 0, L.stream, grids, fmr, weight);
+        hipLaunchKernelGGL(k_fr_fine<float>, grid, dim3(256), 0, L.stream, grids, fmr, weight);
     return hipGetLastError();
 }
 
