@@ -1662,6 +1662,10 @@ extern "C" int bsn_fmr_create(bsn_ctx* ctx, int nlevels, bsn_grid* const* levels
         }
         if (hipMalloc((void**)&f.Uci, 5 * levels[l]->h.fs * 8) != hipSuccess)
             return fail(set_err(BSN_ERR_HIP, "interpolation buffer allocation"));
+        const int64_t box = (int64_t)(f.phi[0] - f.plo[0] + 1) * (f.phi[1] - f.plo[1] + 1) *
+                            (f.phi[2] - f.plo[2] + 1);
+        if (hipMalloc((void**)&f.Ucit, 5 * box * 8) != hipSuccess)
+            return fail(set_err(BSN_ERR_HIP, "interpolation buffer allocation"));
         m->hf.push_back(f);
     }
     if (!m->hf.empty() && upload_vec(&m->d_f, m->hf) != BSN_OK) return fail(BSN_ERR_HIP);
@@ -1675,7 +1679,9 @@ extern "C" int bsn_fmr_destroy(bsn_fmr* m) {
     (void)hipStreamSynchronize(m->ctx->stream);
     graph_drop(m->gexec, m->graph);
     for (auto* P : m->plans) plan_free(P);
-    for (auto& f : m->hf) { (void)hipFree(f.frc); (void)hipFree(f.frf); (void)hipFree(f.Uci); }
+    for (auto& f : m->hf) {
+        (void)hipFree(f.frc); (void)hipFree(f.frf); (void)hipFree(f.Uci); (void)hipFree(f.Ucit);
+    }
     for (auto* t : m->d_fst) (void)hipFree(t);
     for (auto* g : m->lev) g->h.nzkeep = 0;
     (void)hipFree(m->d_f);
@@ -1736,7 +1742,8 @@ static int fmr_prolong(bsn_fmr* m, int l, int all_cells, bool interp,
     const GridDev& Pg = m->lev[l - 1]->h;
     const int64_t total = all_cells ? (int64_t)(C.nx + 2 * BSN_NG) * C.Ny * C.Nz : ghost_total(C);
     HIPCHK(launch_prolong(m->ctx->L(), m->d_all, m->d_f + (l - 1), total,
-                          interp ? m->hf[l - 1].Uci : Pg.U, all_cells, dst));
+                          interp ? m->hf[l - 1].Uci : Pg.U, interp ? m->hf[l - 1].Ucit : nullptr,
+                          all_cells, dst));
     return BSN_OK;
 }
 

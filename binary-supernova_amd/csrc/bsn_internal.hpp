@@ -99,5 +99,6 @@ struct FmrDev {
     double* Ucn;              // parent state at the start of its step
     double* Ucnp1;            // parent state at the end of its RK3 step
     double* Uci;              // parent state interpolated in time (parent layout)
+    double* Ucit;             // the same over the plo..phi box, y fastest, then z, then x
     int plo[3], phi[3];       // padded parent cells any prolongation stencil reads
 };

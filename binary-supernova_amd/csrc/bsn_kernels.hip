@@ -989,7 +989,8 @@ __device__ __forceinline__ void coarse_stencil(int ig_f1, int c_lo, int ratio, i
 // The values go to each of the dst.n buffers in the child's layout, or to
 // the child's state when dst.n is 0.
 __global__ void k_prolong(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr,
-                          const double* __restrict__ Uca, int all_cells, ProlongDst dst) {
+                          const double* __restrict__ Uca, const double* __restrict__ Ucat,
+                          int all_cells, ProlongDst dst) {
     const FmrDev& M = *fmr;
     const GridDev& C = grids[M.parent];
     const GridDev& Fg = grids[M.child];
@@ -998,10 +999,19 @@ __global__ void k_prolong(const GridDev* __restrict__ grids, const FmrDev* __res
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
+        // x-slab ghost cells read the transposed copy when there is one: their
+        // lanes then run along y, where consecutive cells are adjacent in it
+        const bool xt = !all_cells && Ucat && e < 2 * NG * (int64_t)Fg.Ny * Fg.Nz;
         if (all_cells) {
             i = (int)(e % Nfx);
             const int64_t r = e / Nfx;
             j = (int)(r % Fg.Ny); k = (int)(r / Fg.Ny);
+        } else if (xt) {
+            j = (int)(e % Fg.Ny);
+            const int64_t r = e / Fg.Ny;
+            k = (int)(r % Fg.Nz);
+            const int ig = (int)(r / Fg.Nz);
+            i = ig < NG ? ig : Fg.nx + ig;
         } else {
             ghost_decode(Fg, e, i, j, k);
         }
@@ -1010,6 +1020,38 @@ __global__ void k_prolong(const GridDev* __restrict__ grids, const FmrDev* __res
         coarse_stencil(j + 1, M.lo[1], M.ratio, jc0, sj);
         coarse_stencil(k + 1, M.lo[2], M.ratio, kc0, sk);
         const int64_t d = uidx(Fg, i, j, k);
+        if (xt) {
+            // the same terms in the same order, from the transposed box
+            const int bx = M.phi[0] - M.plo[0] + 1, by = M.phi[1] - M.plo[1] + 1;
+            const int bz = M.phi[2] - M.plo[2] + 1;
+            const int64_t bt = (int64_t)bx * by * bz;
+            double val[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int di = -2; di <= 2; ++di) {
+                const int ig = NG + ic0 + di - 1 - M.plo[0];
+                const double wi = M.w[si * 5 + di + 2];
+                for (int dj = -2; dj <= 2; ++dj) {
+                    const int jg = NG + jc0 + dj - 1 - M.plo[1];
+                    const double wij = wi * M.w[sj * 5 + dj + 2];
+                    const int64_t ob = ((int64_t)ig * bz + (NG + kc0 - 3 - M.plo[2])) * by + jg;
+#pragma unroll
+                    for (int dk = 0; dk < 5; ++dk) {
+                        const int64_t o = ob + (int64_t)dk * by;
+                        const double w = wij * M.w[sk * 5 + dk];
+#pragma unroll
+                        for (int q = 0; q < 5; ++q) val[q] += w * Ucat[q * bt + o];
+                    }
+                }
+            }
+            if (dst.n == 0) {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) Fg.U[q * Fg.fs + d] = val[q];
+            }
+            for (int b = 0; b < dst.n; ++b) {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) dst.p[b][q * Fg.fs + d] = val[q];
+            }
+            continue;
+        }
         const int64_t zs = (int64_t)C.Ny * C.pitch;
         // the five fields accumulate side by side; each field's terms are
         // summed in the reference's (di, dj, dk) order
@@ -1057,9 +1099,14 @@ __global__ void k_time_interp(const GridDev* __restrict__ grids, const FmrDev* _
         const int64_t r = e / nx;
         const int j = M.plo[1] + (int)(r % ny), k = M.plo[2] + (int)(r / ny);
         const int64_t o = uidx(C, i, j, k);
+        const int nz = M.phi[2] - M.plo[2] + 1;
+        const int64_t ot = ((int64_t)(i - M.plo[0]) * nz + (k - M.plo[2])) * ny + (j - M.plo[1]);
 #pragma unroll
-        for (int q = 0; q < 5; ++q)
-            M.Uci[q * C.fs + o] = oma * M.Ucn[q * C.fs + o] + alpha * M.Ucnp1[q * C.fs + o];
+        for (int q = 0; q < 5; ++q) {
+            const double v = oma * M.Ucn[q * C.fs + o] + alpha * M.Ucnp1[q * C.fs + o];
+            M.Uci[q * C.fs + o] = v;
+            M.Ucit[q * total + ot] = v;
+        }
     }
 }
 
@@ -1489,10 +1536,10 @@ hipError_t launch_totals(const LaunchCtx& L, const GridDev& G, double* part, int
 }
 
 hipError_t launch_prolong(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total,
-                          const double* Uca, int all_cells, ProlongDst dst) {
+                          const double* Uca, const double* Ucat, int all_cells, ProlongDst dst) {
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_prolong, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids, fmr,
-                       Uca, all_cells, dst);
+                       Uca, Ucat, all_cells, dst);
     return hipGetLastError();
 }
 

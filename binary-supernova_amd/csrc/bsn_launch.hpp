@@ -50,8 +50,11 @@ struct ProlongDst {
     int n;
     double* p[3];
 };
+// Ucat: the interpolated parent box transposed (FmrDev.Ucit) for the x-slab
+// ghost cells, or null to read Uca for every cell.
 hipError_t launch_prolong(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total,
-                          const double* Uca, int all_cells, ProlongDst dst = ProlongDst{0, {}});
+                          const double* Uca, const double* Ucat, int all_cells,
+                          ProlongDst dst = ProlongDst{0, {}});
 hipError_t launch_time_interp(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr,
                               int64_t total, double alpha);
 hipError_t launch_restrict(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total);
