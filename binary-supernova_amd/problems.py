@@ -234,7 +234,7 @@ def solve_poisson_isolated(rho: np.ndarray, dx: float, dy: float, dz: float) -> 
     K = -1.0 / np.sqrt(r2 + eps2)
     pad = np.zeros((Nz, Ny, Nx))
     pad[:nz, :ny, :nx] = rho
-    phi = np.fft.irfftn(np.fft.rfftn(K) * np.fft.rfftn(pad), s=(Nz, Ny, Nx)) * dV
+    phi = np.fft.irfftn(np.fft.rfftn(K) * np.fft.rfftn(pad), s=(Nz, Ny, Nx), axes=(0, 1, 2)) * dV
     return np.ascontiguousarray(phi[:nz, :ny, :nx])
 
 

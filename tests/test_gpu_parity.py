@@ -322,6 +322,34 @@ def test_fmr_split_matches_fp64():
     assert abs(m32 - m64) <= 1e-7 * abs(m64)
 
 
+def test_fmr_full_size_forms_agree():
+    # Config 5 at its full size (192^3 root, 96^3 and 128x128x96 children,
+    # polytrope star, point-mass and self-potential phi), split precision:
+    # two root steps of the fused form, autotuned and replayed as a hipGraph,
+    # give the same bits as the in-place form stepped without a graph.
+    print("fmr full size", flush=True)
+    st = P.fmr_star()
+    dims = [lv[1] for lv in st["levels"]]
+    dxs = [lv[2] for lv in st["levels"]]
+    out = {}
+    for form in ("fused", "inplace"):
+        H = bsn.FMRHierarchy(dims, dxs, st["boxes"], st["ratios"], st["gamma"], "outflow",
+                             *st["floors"], precision="split")
+        H.set_mode(form == "inplace")
+        for g, lv in zip(H.levels, st["levels"]):
+            g.set_state(lv[0])
+        H.prolong_fine_ghosts(1)
+        H.prolong_fine_ghosts(2)
+        H.set_phis(st["phis"])
+        if form == "fused":
+            H.autotune()
+        H.run(0.4, 2, graph=form == "fused")
+        out[form] = [lev.get_state() for lev in H.levels] + [np.array(H.time())]
+        del H
+    for a, b in zip(out["fused"], out["inplace"]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_batch_matches_single_bitwise():
     print("batch", flush=True)
     shapes = [(32, 48, 16), (17, 9, 33), (64, 8, 8)]
