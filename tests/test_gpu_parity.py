@@ -216,33 +216,45 @@ def test_fmr_two_level_with_phi_bitwise(form):
     np.testing.assert_array_equal(G.fine.get_state(), Uf)
 
 
+# three levels: ratios (2, 2), and (4, 2) with a 4:1 first level
+FMR3_CASES = {
+    "r22": dict(boxes=[(5, 12, 5, 12, 6, 11), (5, 12, 4, 13, 5, 8)], ratios=[2, 2],
+                dims=[(16, 16, 12), (16, 20, 8)]),
+    "r42": dict(boxes=[(7, 10, 6, 10, 7, 9), (5, 12, 5, 14, 4, 9)], ratios=[4, 2],
+                dims=[(16, 20, 12), (16, 20, 12)]),
+}
+
+
+@pytest.mark.parametrize("case", sorted(FMR3_CASES))
 @pytest.mark.parametrize("form", FMR_FORMS)
-def test_fmr_three_level_bitwise(form):
-    print(f"fmr 3 levels {form}", flush=True)
+def test_fmr_three_level_bitwise(form, case):
+    print(f"fmr 3 levels {case} {form}", flush=True)
     gm = 5 / 3
     n0, dx0 = 16, 1.0 / 16
     U0 = P.sedov_ic_3d(n0, n0, n0, dx0, dx0, dx0, gm, E_blast=0.5, r_inject=0.2, rho_bg=1.0,
                        P_floor=1e-4, x_offset=-0.5, y_offset=-0.5, z_offset=-0.5)
     O.fill_ghost(U0, "outflow")
-    b1, b2 = (5, 12, 5, 12, 6, 11), (5, 12, 4, 13, 5, 8)
-    d1 = (16, 16, 12)
-    d2 = (16, 20, 8)
-    H = bsn.FMRHierarchy([(n0,) * 3, d1, d2], [(dx0,) * 3, (dx0 / 2,) * 3, (dx0 / 4,) * 3],
-                         [b1, b2], [2, 2], gm, "outflow", 1e-10, 1e-5)
+    cs = FMR3_CASES[case]
+    (b1, b2), (r1, r2) = cs["boxes"], cs["ratios"]
+    d1, d2 = cs["dims"]
+    dx1, dx2 = dx0 / r1, dx0 / (r1 * r2)
+    H = bsn.FMRHierarchy([(n0,) * 3, d1, d2], [(dx0,) * 3, (dx1,) * 3, (dx2,) * 3],
+                         [b1, b2], [r1, r2], gm, "outflow", 1e-10, 1e-5)
     H.set_mode(form == "inplace")
     H.levels[0].set_state(U0)
     H.prolong_all(1)
     H.prolong_all(2)
     U1 = P.empty_state(*d1)
-    O.prolong_all(U1, U0, b1, 2)
+    O.prolong_all(U1, U0, b1, r1)
     U2 = P.empty_state(*d2)
-    O.prolong_all(U2, U1, b2, 2)
+    O.prolong_all(U2, U1, b2, r2)
     np.testing.assert_array_equal(H.levels[1].get_state(), U1)
     np.testing.assert_array_equal(H.levels[2].get_state(), U2)
-    dxs = [(dx0,) * 3, (dx0 / 2,) * 3, (dx0 / 4,) * 3]
+    dxs = [(dx0,) * 3, (dx1,) * 3, (dx2,) * 3]
     for _ in range(2):
         dt = H.cfl_dt(0.4)
-        O.fmr_multilevel_step([U0, U1, U2], dxs, [b1, b2], [2, 2], gm, dt, "outflow", 1e-10, 1e-5)
+        O.fmr_multilevel_step([U0, U1, U2], dxs, [b1, b2], [r1, r2], gm, dt, "outflow", 1e-10,
+                              1e-5)
         H.step(dt)
     for lev, U in zip(H.levels, (U0, U1, U2)):
         np.testing.assert_array_equal(lev.get_state(), U)
