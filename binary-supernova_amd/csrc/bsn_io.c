@@ -553,3 +553,210 @@ done:
     H5Fclose(f);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* two-level FMR checkpoints (src/checkpoint.jl:1-172)                       */
+
+static const char* BOX_ATTR[6] = {"ci_lo", "ci_hi", "cj_lo", "cj_hi", "ck_lo", "ck_hi"};
+static const char* LEVEL[2] = {"coarse", "fine"};
+static const char* N_ATTR[3] = {"nx", "ny", "nz"};
+static const char* D_ATTR[3] = {"dx", "dy", "dz"};
+
+/* One level: /grid/<name> with nx, ny, nz, dx, dy, dz and the dataset U
+ * (Julia's (NQ, nx+2NG, ny+2NG, nz+2NG), so (nz+6, ny+6, nx+6, 5) here),
+ * streamed from the grid g or written from U_host. */
+static int write_level(hid_t gr, int l, bsn_grid* g, const double* U_host, const int64_t n[3],
+                       const double d[3]) {
+    int rc = BSN_OK;
+    struct slab_io S = {{-1, -1, -1, -1, -1}, 4,
+                        {(hsize_t)(n[2] + 2 * NG), (hsize_t)(n[1] + 2 * NG), (hsize_t)(n[0] + 2 * NG), 5}, 1};
+    hid_t sub = H5Gcreate2(gr, LEVEL[l], H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    if (sub < 0) return fail(BSN_ERR_IO, "cannot create group grid/%s", LEVEL[l]);
+    for (int a = 0; a < 3; ++a) TRY(put_i64(sub, N_ATTR[a], n[a]));
+    for (int a = 0; a < 3; ++a) TRY(put_f64(sub, D_ATTR[a], d[a]));
+    if (g) {
+        S.ds[0] = make_dataset(sub, "U", 4, S.dims);
+        if (S.ds[0] < 0) { rc = fail(BSN_ERR_IO, "cannot create dataset grid/%s/U", LEVEL[l]); goto done; }
+        int e = bsn_grid_export_slabs(g, BSN_LAYOUT_REF, 0, slab_cb, &S);
+        if (e != BSN_OK) { rc = stream_error(e, "streaming a level"); goto done; }
+    } else {
+        TRY(write_all(sub, "U", 4, S.dims, U_host));
+    }
+done:
+    if (S.ds[0] >= 0) H5Dclose(S.ds[0]);
+    H5Gclose(sub);
+    return rc;
+}
+
+int bsn_save_checkpoint_fmr(bsn_grid* coarse, bsn_grid* fine, const double* Uc_host,
+                            const double* Uf_host, const int64_t n[6], const double d[6],
+                            const char* path, int64_t step, const double meta[2],
+                            const int64_t box[6], int ratio, const double params[3], const char* bc,
+                            int n_bodies, const double* pos, const double* vel,
+                            const double* props) {
+    if (!path || !n || !d || !meta || !box || !params || !bc || ratio < 1 || n_bodies < 0 ||
+        (n_bodies > 0 && (!pos || !vel || !props)))
+        return fail(BSN_ERR_ARG, "bad arguments");
+    if (!((coarse && fine) || (Uc_host && Uf_host)))
+        return fail(BSN_ERR_ARG, "no state to save (two grids or two host arrays)");
+    if (strcmp(bc, "outflow") != 0 && strcmp(bc, "periodic") != 0)
+        return fail(BSN_ERR_ARG, "bc must be \"outflow\" or \"periodic\"");
+    for (int a = 0; a < 3; ++a) {
+        if (n[a] < 1 || box[2 * a] < 1 || box[2 * a + 1] > n[a] || box[2 * a + 1] < box[2 * a] ||
+            n[3 + a] != (box[2 * a + 1] - box[2 * a] + 1) * ratio)
+            return fail(BSN_ERR_ARG, "the fine level does not cover the box times the ratio (axis %d)", a);
+    }
+    if (coarse && fine) {
+        int64_t gc[3], gf[3];
+        if (bsn_grid_dims(coarse, gc) != BSN_OK || bsn_grid_dims(fine, gf) != BSN_OK)
+            return stream_error(BSN_ERR_ARG, "grid");
+        for (int a = 0; a < 3; ++a)
+            if (gc[a] != n[a] || gf[a] != n[3 + a])
+                return fail(BSN_ERR_ARG, "the grids do not have the stated shapes");
+    }
+    quiet();
+    int rc = BSN_OK;
+    hid_t m = -1, gr = -1;
+    hid_t f = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (f < 0) return fail(BSN_ERR_IO, "cannot create %s", path);
+    m = H5Gcreate2(f, "manifest", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    if (m < 0) { rc = fail(BSN_ERR_IO, "cannot create group manifest"); goto done; }
+    TRY(put_i64(m, "format_version", CHECKPOINT_FORMAT_VERSION));
+    TRY(put_f64(m, "t", meta[0]));
+    TRY(put_i64(m, "step", step));
+    TRY(put_f64(m, "dt_last", meta[1]));
+    gr = H5Gcreate2(f, "grid", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    if (gr < 0) { rc = fail(BSN_ERR_IO, "cannot create group grid"); goto done; }
+    TRY(put_i64(gr, "ratio", ratio));
+    TRY(put_f64(gr, "gamma", params[0]));
+    TRY(put_str(gr, "bc", bc));
+    TRY(put_f64(gr, "rho_floor", params[1]));
+    TRY(put_f64(gr, "P_floor", params[2]));
+    for (int a = 0; a < 6; ++a) TRY(put_i64(gr, BOX_ATTR[a], box[a]));
+    TRY(write_level(gr, 0, coarse && fine ? coarse : NULL, Uc_host, n, d));
+    TRY(write_level(gr, 1, coarse && fine ? fine : NULL, Uf_host, n + 3, d + 3));
+    TRY(write_bodies(f, n_bodies, pos, vel, props));
+done:
+    if (gr >= 0) H5Gclose(gr);
+    if (m >= 0) H5Gclose(m);
+    if (H5Fclose(f) < 0 && rc == BSN_OK) rc = fail(BSN_ERR_IO, "cannot close %s", path);
+    return rc;
+}
+
+/* Opens a two-level checkpoint after checking format_version (as
+ * load_checkpoint, src/checkpoint.jl:128-130) and that both levels exist. */
+static hid_t open_fmr(const char* path) {
+    quiet();
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) { fail(BSN_ERR_IO, "cannot open %s", path); return -1; }
+    hid_t m = H5Gopen2(f, "manifest", H5P_DEFAULT);
+    int64_t fv = -1;
+    int ok = m >= 0 && get_i64(m, "format_version", &fv) == BSN_OK;
+    if (m >= 0) H5Gclose(m);
+    if (ok && fv != CHECKPOINT_FORMAT_VERSION) {
+        H5Fclose(f);
+        fail(BSN_ERR_IO, "checkpoint format_version=%lld, expected %d", (long long)fv,
+             CHECKPOINT_FORMAT_VERSION);
+        return -1;
+    }
+    ok = ok && H5Lexists(f, "grid", H5P_DEFAULT) > 0 && H5Lexists(f, "grid/coarse", H5P_DEFAULT) > 0 &&
+         H5Lexists(f, "grid/fine", H5P_DEFAULT) > 0;
+    if (!ok) {
+        H5Fclose(f);
+        fail(BSN_ERR_IO, "%s is not a two-level checkpoint", path);
+        return -1;
+    }
+    return f;
+}
+
+int bsn_checkpoint_fmr_info(const char* path, int64_t n[6], double d[6], int64_t box[6],
+                            int* ratio, double params[3], char* bc, int bc_len, int* n_bodies) {
+    if (!path || !n || !d || !box || !ratio || !params || !bc || bc_len < 1 || !n_bodies)
+        return fail(BSN_ERR_ARG, "NULL argument");
+    hid_t f = open_fmr(path);
+    if (f < 0) return BSN_ERR_IO;
+    int rc = BSN_OK;
+    int64_t r = 0, nb = 0;
+    hid_t gr = H5Gopen2(f, "grid", H5P_DEFAULT), b = H5Gopen2(f, "bhs", H5P_DEFAULT);
+    hid_t lv[2] = {-1, -1};
+    if (gr < 0 || b < 0) { rc = fail(BSN_ERR_IO, "group grid or bhs missing"); goto done; }
+    TRY(get_i64(gr, "ratio", &r));
+    *ratio = (int)r;
+    TRY(get_f64(gr, "gamma", &params[0]));
+    TRY(get_str(gr, "bc", bc, (size_t)bc_len));
+    TRY(get_f64(gr, "rho_floor", &params[1]));
+    TRY(get_f64(gr, "P_floor", &params[2]));
+    for (int a = 0; a < 6; ++a) TRY(get_i64(gr, BOX_ATTR[a], &box[a]));
+    for (int l = 0; l < 2; ++l) {
+        lv[l] = H5Gopen2(gr, LEVEL[l], H5P_DEFAULT);
+        if (lv[l] < 0) { rc = fail(BSN_ERR_IO, "group grid/%s missing", LEVEL[l]); goto done; }
+        for (int a = 0; a < 3; ++a) TRY(get_i64(lv[l], N_ATTR[a], &n[3 * l + a]));
+        for (int a = 0; a < 3; ++a) TRY(get_f64(lv[l], D_ATTR[a], &d[3 * l + a]));
+    }
+    TRY(get_i64(b, "n", &nb));
+    *n_bodies = (int)nb;
+done:
+    for (int l = 0; l < 2; ++l) if (lv[l] >= 0) H5Gclose(lv[l]);
+    if (gr >= 0) H5Gclose(gr);
+    if (b >= 0) H5Gclose(b);
+    H5Fclose(f);
+    return rc;
+}
+
+int bsn_load_checkpoint_fmr(bsn_grid* coarse, bsn_grid* fine, double* Uc_host, double* Uf_host,
+                            const char* path, int64_t* step, double meta[3], int max_bodies,
+                            int* n_bodies, double* pos, double* vel, double* props) {
+    if (!path || !step || !meta || !n_bodies || !((coarse && fine) || (Uc_host && Uf_host)))
+        return fail(BSN_ERR_ARG, "bad arguments");
+    hid_t f = open_fmr(path);
+    if (f < 0) return BSN_ERR_IO;
+    int rc = BSN_OK;
+    int64_t fv = 0;
+    struct slab_io S = {{-1, -1, -1, -1, -1}, 4, {0, 0, 0, 5}, 0};
+    hid_t lv = -1;
+    hid_t m = H5Gopen2(f, "manifest", H5P_DEFAULT), gr = H5Gopen2(f, "grid", H5P_DEFAULT);
+    if (m < 0 || gr < 0) { rc = fail(BSN_ERR_IO, "group manifest or grid missing"); goto done; }
+    TRY(get_f64(m, "t", &meta[0]));
+    TRY(get_i64(m, "step", step));
+    TRY(get_f64(m, "dt_last", &meta[1]));
+    TRY(get_i64(m, "format_version", &fv));
+    meta[2] = (double)fv;
+    for (int l = 0; l < 2; ++l) {
+        int64_t n[3];
+        lv = H5Gopen2(gr, LEVEL[l], H5P_DEFAULT);
+        if (lv < 0) { rc = fail(BSN_ERR_IO, "group grid/%s missing", LEVEL[l]); goto done; }
+        for (int a = 0; a < 3; ++a) TRY(get_i64(lv, N_ATTR[a], &n[a]));
+        S.dims[0] = (hsize_t)(n[2] + 2 * NG);
+        S.dims[1] = (hsize_t)(n[1] + 2 * NG);
+        S.dims[2] = (hsize_t)(n[0] + 2 * NG);
+        bsn_grid* g = l == 0 ? coarse : fine;
+        if (coarse && fine) {
+            int64_t gn[3];
+            if (bsn_grid_dims(g, gn) != BSN_OK) { rc = stream_error(BSN_ERR_ARG, "grid"); goto done; }
+            if (gn[0] != n[0] || gn[1] != n[1] || gn[2] != n[2]) {
+                rc = fail(BSN_ERR_ARG, "checkpoint %s level is %lldx%lldx%lld, grid is %lldx%lldx%lld",
+                          LEVEL[l], (long long)n[0], (long long)n[1], (long long)n[2],
+                          (long long)gn[0], (long long)gn[1], (long long)gn[2]);
+                goto done;
+            }
+            S.ds[0] = open_checked(lv, "U", 4, S.dims);
+            if (S.ds[0] < 0) { rc = BSN_ERR_IO; goto done; }
+            int e = bsn_grid_import_slabs(g, BSN_LAYOUT_REF, 0, slab_cb, &S);
+            H5Dclose(S.ds[0]);
+            S.ds[0] = -1;
+            if (e != BSN_OK) { rc = stream_error(e, "streaming a level"); goto done; }
+        } else {
+            TRY(read_all(lv, "U", 4, S.dims, l == 0 ? Uc_host : Uf_host));
+        }
+        H5Gclose(lv);
+        lv = -1;
+    }
+    TRY(read_bodies(f, max_bodies, n_bodies, pos, vel, props));
+done:
+    if (S.ds[0] >= 0) H5Dclose(S.ds[0]);
+    if (lv >= 0) H5Gclose(lv);
+    if (gr >= 0) H5Gclose(gr);
+    if (m >= 0) H5Gclose(m);
+    H5Fclose(f);
+    return rc;
+}

@@ -1,7 +1,8 @@
 """Snapshot and checkpoint files (HDF5, the reference's layout) streamed
 between a GPU-resident grid and disk through libbsn_io.so.
 
-Mirrors write_snapshot / read_snapshot (src/io.jl:29-84) and
+Mirrors write_snapshot / read_snapshot (src/io.jl:29-84), save_checkpoint /
+load_checkpoint of the two-level FMR grid (src/checkpoint.jl:1-172) and
 save_checkpoint_uniform / load_checkpoint_uniform (src/checkpoint.jl:244-318).
 A Grid argument streams its state through pinned slabs (the device packs the
 file layout); a numpy array in the reference layout is written from the host.
@@ -22,7 +23,8 @@ IO_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbsn_i
 EXPORTS = [
     "bsn_io_last_error", "bsn_write_snapshot", "bsn_write_snapshot_host", "bsn_snapshot_info",
     "bsn_read_snapshot", "bsn_read_snapshot_host", "bsn_save_checkpoint_uniform",
-    "bsn_checkpoint_uniform_info", "bsn_load_checkpoint_uniform",
+    "bsn_checkpoint_uniform_info", "bsn_load_checkpoint_uniform", "bsn_save_checkpoint_fmr",
+    "bsn_checkpoint_fmr_info", "bsn_load_checkpoint_fmr",
 ]
 
 _io = None
@@ -31,6 +33,8 @@ _pd, _pi64 = C.POINTER(C.c_double), C.POINTER(C.c_int64)
 
 GridParams = namedtuple("GridParams", "nx ny nz dx dy dz gamma x0 y0 z0 rho_floor P_floor")
 CheckpointMeta = namedtuple("CheckpointMeta", "t step dt_last t_snap t_bh2_sink_on format_version")
+FMRCheckpointMeta = namedtuple("FMRCheckpointMeta", "t step dt_last format_version")
+FMRGridInfo = namedtuple("FMRGridInfo", "dims dxs box ratio gamma bc rho_floor P_floor")
 
 
 def load_io():
@@ -54,6 +58,12 @@ def load_io():
         "bsn_checkpoint_uniform_info": ([_s, _pi64, C.POINTER(_i)], _i),
         "bsn_load_checkpoint_uniform": ([_p, _p, _s, _pi64, _pd, _pd, _i, C.POINTER(_i), _p, _p,
                                          _p], _i),
+        "bsn_save_checkpoint_fmr": ([_p, _p, _p, _p, _pi64, _pd, _s, _i64, _pd, _pi64, _i, _pd, _s,
+                                     _i, _p, _p, _p], _i),
+        "bsn_checkpoint_fmr_info": ([_s, _pi64, _pd, _pi64, C.POINTER(_i), _pd, C.c_char_p, _i,
+                                     C.POINTER(_i)], _i),
+        "bsn_load_checkpoint_fmr": ([_p, _p, _p, _p, _s, _pi64, _pd, _i, C.POINTER(_i), _p, _p,
+                                     _p], _i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -166,3 +176,118 @@ def load_checkpoint_uniform(path, grid: Grid | None = None):
     gp = GridParams(nx, ny, nz, *params)
     m = CheckpointMeta(meta[0], step.value, meta[1], meta[2], meta[3], int(meta[4]))
     return (grid if grid is not None else U), bhs, gp, m
+
+
+def _bodies(bhs):
+    n = len(bhs)
+    pos = np.ascontiguousarray([[float(c) for c in b.pos] for b in bhs], dtype=np.float64).reshape(n, 3)
+    vel = np.ascontiguousarray([[float(c) for c in b.vel] for b in bhs], dtype=np.float64).reshape(n, 3)
+    props = np.ascontiguousarray([[b.mass, b.eps, b.c_code, b.r_floor] for b in bhs],
+                                 dtype=np.float64).reshape(n, 4)
+    return n, pos, vel, props
+
+
+def _bc_name(mask: int) -> str:
+    if mask not in (0, 7):
+        raise ValueError("the two-level checkpoint stores one bc for all axes")
+    return "periodic" if mask == 7 else "outflow"
+
+
+def save_checkpoint(path, G, bhs, *, t=0.0, step=0, dt_last=0.0):
+    """save_checkpoint(path, G, bhs; t, step, dt_last) (src/checkpoint.jl:33-117)
+    of a two-level FMRHierarchy: both levels' states stream from the device
+    into /grid/coarse/U and /grid/fine/U."""
+    from .fmr3d import FMRHierarchy
+    if not isinstance(G, FMRHierarchy) or len(G.levels) != 2:
+        raise ValueError("the checkpoint format holds a two-level FMR grid")
+    n = (C.c_int64 * 6)(*G.dims[0], *G.dims[1])
+    d = (C.c_double * 6)(*G.dxs[0], *G.dxs[1])
+    box = (C.c_int64 * 6)(*G.boxes[0])
+    params = (C.c_double * 3)(G.gamma, G.rho_floor, G.P_floor)
+    meta = (C.c_double * 2)(float(t), float(dt_last))
+    nb, pos, vel, props = _bodies(bhs)
+    _call("bsn_save_checkpoint_fmr", G.levels[0].handle, G.levels[1].handle, None, None, n, d,
+          _path(path), int(step), meta, box, int(G.ratios[0]), params,
+          _bc_name(G.bc).encode(), nb, _ptr(pos), _ptr(vel), _ptr(props))
+    return path
+
+
+def checkpoint_fmr_info(path):
+    """(FMRGridInfo, number of bodies) of a two-level checkpoint."""
+    n = (C.c_int64 * 6)()
+    d = (C.c_double * 6)()
+    box = (C.c_int64 * 6)()
+    ratio = C.c_int()
+    params = (C.c_double * 3)()
+    bc = C.create_string_buffer(32)
+    nb = C.c_int()
+    _call("bsn_checkpoint_fmr_info", _path(path), n, d, box, C.byref(ratio), params, bc, 32,
+          C.byref(nb))
+    info = FMRGridInfo(dims=[tuple(n[:3]), tuple(n[3:])], dxs=[tuple(d[:3]), tuple(d[3:])],
+                       box=tuple(box), ratio=ratio.value, gamma=params[0],
+                       bc=bc.value.decode(), rho_floor=params[1], P_floor=params[2])
+    return info, nb.value
+
+
+def load_checkpoint(path, G=None, *, precision="fp64"):
+    """load_checkpoint(path) (src/checkpoint.jl:119-172): (G, bhs, meta).
+    Without G a two-level FMRHierarchy of the stored shape is built first;
+    both levels' saved states (the fine one as saved, not prolonged again)
+    are streamed onto the device."""
+    from .fmr3d import FMRHierarchy
+    info, nb = checkpoint_fmr_info(path)
+    if G is None:
+        G = FMRHierarchy(info.dims, info.dxs, [info.box], [info.ratio], info.gamma, info.bc,
+                         info.rho_floor, info.P_floor, precision=precision)
+    elif len(G.levels) != 2:
+        raise ValueError("the checkpoint format holds a two-level FMR grid")
+    step = C.c_int64()
+    meta = (C.c_double * 3)()
+    got = C.c_int()
+    pos, vel, props = np.zeros((max(nb, 1), 3)), np.zeros((max(nb, 1), 3)), np.zeros((max(nb, 1), 4))
+    _call("bsn_load_checkpoint_fmr", G.levels[0].handle, G.levels[1].handle, None, None,
+          _path(path), C.byref(step), meta, max(nb, 1), C.byref(got), _ptr(pos), _ptr(vel),
+          _ptr(props))
+    bhs = [BlackHole(pos=list(pos[i]), vel=list(vel[i]), mass=float(props[i, 0]),
+                     eps=float(props[i, 1]), c_code=float(props[i, 2]),
+                     r_floor=float(props[i, 3])) for i in range(got.value)]
+    return G, bhs, FMRCheckpointMeta(meta[0], step.value, meta[1], int(meta[2]))
+
+
+def save_checkpoint_host(path, Uc, Uf, bhs, *, dims, dxs, box, ratio, gamma, bc="outflow",
+                         rho_floor=0.0, P_floor=0.0, t=0.0, step=0, dt_last=0.0):
+    """The same file from host arrays in the reference layout (coarse and
+    fine states with ghosts)."""
+    (nxc, nyc, nzc), (nxf, nyf, nzf) = dims
+    Uc = np.ascontiguousarray(Uc, dtype=np.float64)
+    Uf = np.ascontiguousarray(Uf, dtype=np.float64)
+    if Uc.shape != (nzc + 2 * NG, nyc + 2 * NG, nxc + 2 * NG, 5) or \
+            Uf.shape != (nzf + 2 * NG, nyf + 2 * NG, nxf + 2 * NG, 5):
+        raise ValueError("state shapes do not match dims with ghosts")
+    n = (C.c_int64 * 6)(nxc, nyc, nzc, nxf, nyf, nzf)
+    d = (C.c_double * 6)(*dxs[0], *dxs[1])
+    nb, pos, vel, props = _bodies(bhs)
+    _call("bsn_save_checkpoint_fmr", None, None, _ptr(Uc), _ptr(Uf), n, d, _path(path), int(step),
+          (C.c_double * 2)(float(t), float(dt_last)), (C.c_int64 * 6)(*box), int(ratio),
+          (C.c_double * 3)(gamma, rho_floor, P_floor), bc.encode(), nb, _ptr(pos), _ptr(vel),
+          _ptr(props))
+    return path
+
+
+def load_checkpoint_host(path):
+    """(Uc, Uf, bhs, FMRGridInfo, meta) of a two-level checkpoint, as host
+    arrays in the reference layout."""
+    info, nb = checkpoint_fmr_info(path)
+    (nxc, nyc, nzc), (nxf, nyf, nzf) = info.dims
+    Uc = np.empty((nzc + 2 * NG, nyc + 2 * NG, nxc + 2 * NG, 5))
+    Uf = np.empty((nzf + 2 * NG, nyf + 2 * NG, nxf + 2 * NG, 5))
+    step = C.c_int64()
+    meta = (C.c_double * 3)()
+    got = C.c_int()
+    pos, vel, props = np.zeros((max(nb, 1), 3)), np.zeros((max(nb, 1), 3)), np.zeros((max(nb, 1), 4))
+    _call("bsn_load_checkpoint_fmr", None, None, _ptr(Uc), _ptr(Uf), _path(path), C.byref(step),
+          meta, max(nb, 1), C.byref(got), _ptr(pos), _ptr(vel), _ptr(props))
+    bhs = [BlackHole(pos=list(pos[i]), vel=list(vel[i]), mass=float(props[i, 0]),
+                     eps=float(props[i, 1]), c_code=float(props[i, 2]),
+                     r_floor=float(props[i, 3])) for i in range(got.value)]
+    return Uc, Uf, bhs, info, FMRCheckpointMeta(meta[0], step.value, meta[1], int(meta[2]))

@@ -22,7 +22,8 @@ export HIPGrid, euler3d_step!, euler3d_rhs!, cfl_dt_3d, apply_floors_3d!,
        add_sink_sources!, set_point_masses!, set_sinks!, solve_poisson_isolated,
        set_self_gravity!, gas_force_on_bh, accrete!, gas_diagnostics, gas_energy_total,
        gas_kinetic_total, gas_momentum_total, gas_angular_momentum_total, bound_gas_mass,
-       write_snapshot, read_snapshot!, save_checkpoint_uniform, load_checkpoint_uniform!
+       write_snapshot, read_snapshot!, save_checkpoint_uniform, load_checkpoint_uniform!,
+       save_checkpoint, load_checkpoint!
 
 const libbsn = joinpath(@__DIR__, "..", "libbsn_hip.so")
 const libbsn_io = joinpath(@__DIR__, "..", "libbsn_io.so")
@@ -430,6 +431,48 @@ function load_checkpoint_uniform!(U::HIPGrid, path::AbstractString; max_bodies::
     m = (; t = meta[1], step = step[], dt_last = meta[2], t_snap = meta[3],
          t_bh2_sink_on = meta[4], format_version = Int(meta[5]))
     return bodies, grid, m
+end
+
+"""save_checkpoint(path, G, bhs; t, step, dt_last, box, ratio, dxs, γ, bc, ρ_floor, P_floor)
+(src/checkpoint.jl:33-117) for a two-level HIPFMR: both levels stream from
+the device.  box = (ci_lo, ci_hi, cj_lo, cj_hi, ck_lo, ck_hi); dxs = the
+(dx, dy, dz) of the coarse and the fine level."""
+function save_checkpoint(path::AbstractString, G::HIPFMR, bhs; t::Real = 0.0,
+                         step::Integer = 0, dt_last::Real = 0.0, box::NTuple{6,Int},
+                         ratio::Integer, dxs::NTuple{2,NTuple{3,Float64}}, γ::Real,
+                         bc::Symbol = :outflow, ρ_floor::Real = 0.0, P_floor::Real = 0.0)
+    length(G.levels) == 2 || error("the checkpoint format holds a two-level FMR grid")
+    c, f = G.levels
+    n = Int64[c.nx, c.ny, c.nz, f.nx, f.ny, f.nz]
+    d = Float64[dxs[1]..., dxs[2]...]
+    props = Float64[p for bh in bhs for p in (bh.mass, bh.eps, bh.c_code, bh.r_floor)]
+    checkio(ccall((:bsn_save_checkpoint_fmr, libbsn_io), Cint,
+                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Ptr{Float64},
+                   Cstring, Int64, Ptr{Float64}, Ptr{Int64}, Cint, Ptr{Float64}, Cstring, Cint,
+                   Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                  c.handle, f.handle, C_NULL, C_NULL, n, d, path, step, Float64[t, dt_last],
+                  Int64[box...], Cint(ratio), Float64[γ, ρ_floor, P_floor], String(bc),
+                  Cint(length(bhs)), bh_pos(bhs), bh_vel(bhs), props))
+    return path
+end
+
+"""load_checkpoint!(G, path) (src/checkpoint.jl:119-172): both saved level
+states into the two-level HIPFMR G (which must have the stored shapes);
+returns (bodies as (pos, vel, mass, eps, c_code, r_floor) tuples, meta)."""
+function load_checkpoint!(G::HIPFMR, path::AbstractString; max_bodies::Int = 8)
+    length(G.levels) == 2 || error("the checkpoint format holds a two-level FMR grid")
+    step = Ref{Int64}(0)
+    meta = zeros(3); nb = Ref{Cint}(0)
+    pos = zeros(3 * max_bodies); vel = zeros(3 * max_bodies); props = zeros(4 * max_bodies)
+    checkio(ccall((:bsn_load_checkpoint_fmr, libbsn_io), Cint,
+                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cstring, Ref{Int64},
+                   Ptr{Float64}, Cint, Ref{Cint}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                  G.levels[1].handle, G.levels[2].handle, C_NULL, C_NULL, path, step, meta,
+                  Cint(max_bodies), nb, pos, vel, props))
+    bodies = [(pos[3i-2:3i], vel[3i-2:3i], props[4i-3], props[4i-2], props[4i-1], props[4i])
+              for i in 1:nb[]]
+    return bodies, (; t = meta[1], step = step[], dt_last = meta[2],
+                    format_version = Int(meta[3]))
 end
 
 end # module

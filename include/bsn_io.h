@@ -60,6 +60,36 @@ int bsn_load_checkpoint_uniform(bsn_grid* g, double* U_host, const char* path, i
                                 double meta[5], double params[9], int max_bodies, int* n_bodies,
                                 double* pos, double* vel, double* props);
 
+/* save_checkpoint(path, G, bhs; t, step, dt_last) (src/checkpoint.jl:33-117):
+ * the two-level FMR grid of the reference (format version 1).  The states
+ * with ghosts stream from the grids coarse and fine, or come from the host
+ * arrays Uc_host and Uf_host (reference layout) when the grids are NULL.
+ *   n      = coarse (nx, ny, nz), fine (nx, ny, nz)
+ *   d      = coarse (dx, dy, dz), fine (dx, dy, dz)
+ *   meta   = (t, dt_last)
+ *   box    = (ci_lo, ci_hi, cj_lo, cj_hi, ck_lo, ck_hi), 1-based coarse cells
+ *   params = (gamma, rho_floor, P_floor); bc = "outflow" or "periodic"
+ *   bodies as for bsn_save_checkpoint_uniform.
+ * BSN_ERR_ARG when the fine shape is not the box times the ratio. */
+int bsn_save_checkpoint_fmr(bsn_grid* coarse, bsn_grid* fine, const double* Uc_host,
+                            const double* Uf_host, const int64_t n[6], const double d[6],
+                            const char* path, int64_t step, const double meta[2],
+                            const int64_t box[6], int ratio, const double params[3], const char* bc,
+                            int n_bodies, const double* pos, const double* vel,
+                            const double* props);
+/* The grid description of a two-level checkpoint (shapes and cell sizes as
+ * n and d above, box, ratio, params, the bc string into bc[bc_len]) and its
+ * body count; BSN_ERR_IO unless format_version == 1 and both levels exist. */
+int bsn_checkpoint_fmr_info(const char* path, int64_t n[6], double d[6], int64_t box[6],
+                            int* ratio, double params[3], char* bc, int bc_len, int* n_bodies);
+/* load_checkpoint(path) (src/checkpoint.jl:119-172): both levels' saved states
+ * (the fine one as saved, not prolonged again) into the grids, which must have
+ * the stored shapes, or into the host arrays when the grids are NULL;
+ *   meta = (t, dt_last, format_version); up to max_bodies bodies. */
+int bsn_load_checkpoint_fmr(bsn_grid* coarse, bsn_grid* fine, double* Uc_host, double* Uf_host,
+                            const char* path, int64_t* step, double meta[3], int max_bodies,
+                            int* n_bodies, double* pos, double* vel, double* props);
+
 #ifdef __cplusplus
 }
 #endif

@@ -139,3 +139,42 @@ def test_uniform_checkpoint_from_and_to_the_device(tmp_path):
     q = tmp_path / "ckpt_host.h5"
     bio.save_checkpoint_uniform(q, U, bhs, **kw)
     np.testing.assert_array_equal(bio.load_checkpoint_uniform(q)[0], Uh)
+
+
+def test_fmr_checkpoint_from_and_to_the_device(tmp_path):
+    # save_checkpoint / load_checkpoint of a two-level grid (src/checkpoint.jl
+    # :33-172) streamed from and to the device: a stepped hierarchy, saved and
+    # loaded into a hierarchy built from the file, holds the same bits, and
+    # both step on to the same bits (the restart is faithful)
+    from oracle import oracle as O
+    from binary_supernova_amd import problems as P
+    print("fmr checkpoint", flush=True)
+    gm, nc, L, r = 5 / 3, 8, 0.5, 2
+    dxc = 2 * L / nc
+    box = (3, 6, 3, 6, 3, 6)
+    Uc = P.sedov_ic_3d(nc, nc, nc, dxc, dxc, dxc, gm, E_blast=1.0, r_inject=0.15, rho_bg=1.0,
+                       P_floor=1e-5, x_offset=-L, y_offset=-L, z_offset=-L)
+    O.fill_ghost(Uc, "outflow")
+    G = bsn.FMRGrid3D(Uc, nc, nc, nc, dxc, dxc, dxc, *box, r, gm, bc="outflow",
+                      rho_floor=1e-10, P_floor=1e-5)
+    for _ in range(2):
+        G.step(G.cfl_dt(0.4))
+    bhs = [bsn.BlackHole(pos=[0.1, 0.0, 0.0], vel=[0.0, 0.2, 0.0], mass=0.3, eps=1e-3,
+                         c_code=20.0, r_floor=1e-3)]
+    p = tmp_path / "fmr.h5"
+    bio.save_checkpoint(p, G, bhs, t=0.5, step=2, dt_last=1e-3)
+    Uc_f, Uf_f, _, info, _ = bio.load_checkpoint_host(p)
+    np.testing.assert_array_equal(Uc_f, G.levels[0].get_state())
+    np.testing.assert_array_equal(Uf_f, G.levels[1].get_state())
+    H, bhs2, meta = bio.load_checkpoint(p)
+    assert meta == (0.5, 2, 1e-3, 1) and len(bhs2) == 1 and bhs2[0].pos == bhs[0].pos
+    assert H.boxes == [box] and H.ratios == [r] and H.dims == G.dims
+    for a, b in zip(G.levels, H.levels):
+        np.testing.assert_array_equal(b.get_state(), a.get_state())
+    for _ in range(2):
+        dt = G.cfl_dt(0.4)
+        assert H.cfl_dt(0.4) == dt
+        G.step(dt)
+        H.step(dt)
+    for a, b in zip(G.levels, H.levels):
+        np.testing.assert_array_equal(b.get_state(), a.get_state())

@@ -171,3 +171,98 @@ def test_a_snapshot_is_not_a_uniform_checkpoint(tmp_path):
         bio.load_checkpoint_uniform(p)
     with pytest.raises(bsn.BsnError, match="cannot open"):
         bio.read_snapshot(tmp_path / "missing.h5")
+
+
+def _fmr_reference_case():
+    # test/test_checkpoint.jl:4-34: coarse 8^3 (dx = 1/8) with rho = 1 + 0.1 r^2
+    # and E = 1/(gamma - 1) on the active cells, fine region = coarse cells
+    # 4:5 in each axis at ratio 4, prolonged and then scaled by 1.01 so the
+    # saved fine state differs from a re-prolongation
+    from oracle import oracle as O
+    g = 5 / 3
+    n, r, box = 8, 4, (4, 5, 4, 5, 4, 5)
+    Uc = np.zeros((n + 2 * NG, n + 2 * NG, n + 2 * NG, 5))
+    c = (np.arange(n) + 0.5) / n - 0.5
+    r2 = c[:, None, None] ** 2 + c[None, :, None] ** 2 + c[None, None, :] ** 2
+    Uc[NG:-NG, NG:-NG, NG:-NG, 0] = 1.0 + 0.1 * r2
+    Uc[NG:-NG, NG:-NG, NG:-NG, 4] = 1.0 / (g - 1.0)
+    Uf = np.zeros_like(Uc)
+    O.prolong_all(Uf, Uc, box, r)
+    Uf *= 1.01
+    B = bsn.BlackHole
+    bhs = [B(pos=[-0.5, 0.0, 0.0], vel=[0.0, -0.5, 0.0], mass=0.5, eps=1e-3, c_code=20.0,
+             r_floor=1e-3),
+           B(pos=[0.5, 0.1, 0.0], vel=[0.0, 0.5, 0.0], mass=0.5, eps=1e-3, c_code=20.0,
+             r_floor=1e-3)]
+    kw = dict(dims=[(n, n, n), (8, 8, 8)], dxs=[(1 / n,) * 3, (1 / (n * r),) * 3], box=box,
+              ratio=r, gamma=g, bc="outflow", rho_floor=1e-12, P_floor=1e-12)
+    return Uc, Uf, bhs, kw
+
+
+def test_checkpoint_fmr_round_trip(tmp_path):
+    # test/test_checkpoint.jl:36-63: every field back bit for bit
+    Uc, Uf, bhs, kw = _fmr_reference_case()
+    p = tmp_path / "ckpt.h5"
+    bio.save_checkpoint_host(p, Uc, Uf, bhs, t=0.25, step=42, dt_last=0.001, **kw)
+    Uc2, Uf2, bhs2, info, meta = bio.load_checkpoint_host(p)
+    assert meta == (0.25, 42, 0.001, 1)
+    assert (info.ratio, info.gamma, info.bc, info.rho_floor, info.P_floor) == \
+        (4, 5 / 3, "outflow", 1e-12, 1e-12)
+    assert info.box == (4, 5, 4, 5, 4, 5)
+    assert info.dims == [(8, 8, 8), (8, 8, 8)] and info.dxs == kw["dxs"]
+    assert np.array_equal(Uc2, Uc) and np.array_equal(Uf2, Uf)
+    assert len(bhs2) == 2
+    for b, b2 in zip(bhs, bhs2):
+        assert (b2.pos, b2.vel, b2.mass, b2.eps, b2.c_code, b2.r_floor) == \
+            (b.pos, b.vel, b.mass, b.eps, b.c_code, b.r_floor)
+
+
+def test_checkpoint_fmr_with_no_bodies(tmp_path):
+    # test/test_checkpoint.jl:66-80: an empty body list, default t and step
+    Uc, Uf, _, kw = _fmr_reference_case()
+    p = tmp_path / "ckpt_empty.h5"
+    bio.save_checkpoint_host(p, Uc, Uf, [], **kw)
+    _, _, bhs2, _, meta = bio.load_checkpoint_host(p)
+    assert bhs2 == [] and meta.t == 0.0 and meta.step == 0
+
+
+def test_checkpoint_fmr_layout(tmp_path):
+    # the layout of src/checkpoint.jl:3-28, read with h5dump
+    Uc, Uf, bhs, kw = _fmr_reference_case()
+    p = str(tmp_path / "ckpt.h5")
+    bio.save_checkpoint_host(p, Uc, Uf, bhs, t=0.25, step=42, dt_last=0.001, **kw)
+    h = header(p)
+    F, I = "H5T_IEEE_F64LE", "H5T_STD_I64LE"
+    for a, t in (("format_version", I), ("t", F), ("step", I), ("dt_last", F)):
+        assert h["manifest/" + a] == ("ATTRIBUTE", t, ()), a
+    assert "manifest/kind" not in h
+    for a in ("ratio", "ci_lo", "ci_hi", "cj_lo", "cj_hi", "ck_lo", "ck_hi"):
+        assert h["grid/" + a] == ("ATTRIBUTE", I, ()), a
+    for a in ("gamma", "rho_floor", "P_floor"):
+        assert h["grid/" + a] == ("ATTRIBUTE", F, ()), a
+    assert '"outflow"' in h5dump("-a", "/grid/bc", p)
+    for lev, U in (("coarse", Uc), ("fine", Uf)):
+        for a in ("nx", "ny", "nz"):
+            assert h[f"grid/{lev}/{a}"] == ("ATTRIBUTE", I, ())
+        for a in ("dx", "dy", "dz"):
+            assert h[f"grid/{lev}/{a}"] == ("ATTRIBUTE", F, ())
+        assert h[f"grid/{lev}/U"] == ("DATASET", F, (14, 14, 14, 5))
+        np.testing.assert_array_equal(h5_array(p, f"/grid/{lev}/U", U.shape, tmp_path), U)
+    assert h["bhs/pos"] == ("DATASET", F, (2, 3))
+
+
+def test_fmr_and_uniform_checkpoints_are_not_cross_loadable(tmp_path):
+    Uc, Uf, bhs, kw = _fmr_reference_case()
+    pf = tmp_path / "fmr.h5"
+    bio.save_checkpoint_host(pf, Uc, Uf, bhs, **kw)
+    with pytest.raises(bsn.BsnError, match="kind"):
+        bio.load_checkpoint_uniform(pf)
+    U, bhs_u, kwu = _reference_case()
+    pu = tmp_path / "uniform.h5"
+    bio.save_checkpoint_uniform(pu, U, bhs_u, **kwu)
+    with pytest.raises(bsn.BsnError, match="two-level"):
+        bio.load_checkpoint_host(pu)
+    # a fine level that is not the box times the ratio is refused
+    bad = dict(kw, box=(4, 6, 4, 5, 4, 5))
+    with pytest.raises(bsn.BsnError, match="ratio"):
+        bio.save_checkpoint_host(tmp_path / "bad.h5", Uc, Uf, bhs, **bad)
