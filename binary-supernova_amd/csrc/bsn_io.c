@@ -760,3 +760,167 @@ done:
     H5Fclose(f);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* body trajectories (src/io.jl:86-161)                                      */
+/* "time" (N), and per body bh<i>/mass (N), bh<i>/rsink (N), bh<i>/pos and
+ * bh<i>/vel (Julia's (N, 3), so (3, N) here), all extendable along N in
+ * chunks of 128 records. */
+
+static hid_t make_extendable(hid_t loc, const char* name, int rank) {
+    hsize_t dims[2] = {0, 0}, maxd[2], chunk[2];
+    if (rank == 1) { maxd[0] = H5S_UNLIMITED; chunk[0] = 128; }
+    else { dims[0] = 3; maxd[0] = 3; maxd[1] = H5S_UNLIMITED; chunk[0] = 3; chunk[1] = 128; }
+    hid_t sp = H5Screate_simple(rank, dims, maxd);
+    hid_t pl = H5Pcreate(H5P_DATASET_CREATE);
+    H5Pset_chunk(pl, rank, chunk);
+    hid_t d = H5Dcreate2(loc, name, H5T_IEEE_F64LE, sp, H5P_DEFAULT, pl, H5P_DEFAULT);
+    H5Pclose(pl);
+    H5Sclose(sp);
+    return d;
+}
+
+int bsn_init_trajectory_file(const char* filename, int n_bh) {
+    if (!filename || n_bh < 0) return fail(BSN_ERR_ARG, "bad arguments");
+    quiet();
+    static const char* DS[4] = {"mass", "rsink", "pos", "vel"};
+    int rc = BSN_OK;
+    hid_t f = H5Fcreate(filename, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (f < 0) return fail(BSN_ERR_IO, "cannot create %s", filename);
+    hid_t d = make_extendable(f, "time", 1);
+    if (d < 0) { rc = fail(BSN_ERR_IO, "cannot create dataset time"); goto done; }
+    H5Dclose(d);
+    for (int i = 1; i <= n_bh && rc == BSN_OK; ++i) {
+        char name[32];
+        snprintf(name, sizeof name, "bh%d", i);
+        hid_t g = H5Gcreate2(f, name, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+        if (g < 0) { rc = fail(BSN_ERR_IO, "cannot create group %s", name); break; }
+        for (int k = 0; k < 4 && rc == BSN_OK; ++k) {
+            d = make_extendable(g, DS[k], k < 2 ? 1 : 2);
+            if (d < 0) rc = fail(BSN_ERR_IO, "cannot create dataset %s/%s", name, DS[k]);
+            else H5Dclose(d);
+        }
+        H5Gclose(g);
+    }
+done:
+    if (H5Fclose(f) < 0 && rc == BSN_OK) rc = fail(BSN_ERR_IO, "cannot close %s", filename);
+    return rc;
+}
+
+/* Grows dataset d by one record (index n) and writes v (1 or 3 values). */
+static int append_record(hid_t d, int rank, hsize_t n, const double* v) {
+    hsize_t ext[2], start[2], cnt[2];
+    if (rank == 1) { ext[0] = n + 1; start[0] = n; cnt[0] = 1; }
+    else { ext[0] = 3; ext[1] = n + 1; start[0] = 0; start[1] = n; cnt[0] = 3; cnt[1] = 1; }
+    if (H5Dset_extent(d, ext) < 0) return 1;
+    hid_t fs = H5Dget_space(d);
+    hsize_t m = rank == 1 ? 1 : 3;
+    hid_t ms = H5Screate_simple(1, &m, NULL);
+    herr_t e = H5Sselect_hyperslab(fs, H5S_SELECT_SET, start, NULL, cnt, NULL);
+    if (e >= 0) e = H5Dwrite(d, H5T_NATIVE_DOUBLE, ms, fs, H5P_DEFAULT, v);
+    H5Sclose(ms);
+    H5Sclose(fs);
+    return e < 0;
+}
+
+int bsn_append_trajectory(const char* filename, double t, int n_bh, const double* mass,
+                          const double* rsink, const double* pos, const double* vel) {
+    if (!filename || n_bh < 0 || (n_bh > 0 && (!mass || !rsink || !pos || !vel)))
+        return fail(BSN_ERR_ARG, "bad arguments");
+    quiet();
+    int rc = BSN_OK;
+    hid_t f = H5Fopen(filename, H5F_ACC_RDWR, H5P_DEFAULT);
+    if (f < 0) return fail(BSN_ERR_IO, "cannot open %s", filename);
+    hid_t d = H5Dopen2(f, "time", H5P_DEFAULT);
+    if (d < 0) { rc = fail(BSN_ERR_IO, "dataset time missing"); goto done; }
+    hid_t sp = H5Dget_space(d);
+    hsize_t n = 0;
+    H5Sget_simple_extent_dims(sp, &n, NULL);
+    H5Sclose(sp);
+    if (append_record(d, 1, n, &t)) rc = fail(BSN_ERR_IO, "cannot extend time");
+    H5Dclose(d);
+    for (int i = 0; i < n_bh && rc == BSN_OK; ++i) {
+        char name[32];
+        snprintf(name, sizeof name, "bh%d", i + 1);
+        hid_t g = H5Gopen2(f, name, H5P_DEFAULT);
+        if (g < 0) { rc = fail(BSN_ERR_IO, "group %s missing", name); break; }
+        const double* src[4] = {mass + i, rsink + i, pos + 3 * i, vel + 3 * i};
+        static const char* DS[4] = {"mass", "rsink", "pos", "vel"};
+        for (int k = 0; k < 4 && rc == BSN_OK; ++k) {
+            hid_t dk = H5Dopen2(g, DS[k], H5P_DEFAULT);
+            if (dk < 0 || append_record(dk, k < 2 ? 1 : 2, n, src[k]))
+                rc = fail(BSN_ERR_IO, "cannot extend %s/%s", name, DS[k]);
+            if (dk >= 0) H5Dclose(dk);
+        }
+        H5Gclose(g);
+    }
+done:
+    if (H5Fclose(f) < 0 && rc == BSN_OK) rc = fail(BSN_ERR_IO, "cannot close %s", filename);
+    return rc;
+}
+
+int bsn_trajectory_info(const char* filename, int* n_bh, int64_t* n_records) {
+    if (!filename || !n_bh || !n_records) return fail(BSN_ERR_ARG, "NULL argument");
+    quiet();
+    hid_t f = H5Fopen(filename, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return fail(BSN_ERR_IO, "cannot open %s", filename);
+    int rc = BSN_OK;
+    hid_t d = H5Dopen2(f, "time", H5P_DEFAULT);
+    if (d < 0) {
+        rc = fail(BSN_ERR_IO, "dataset time missing");
+    } else {
+        hid_t sp = H5Dget_space(d);
+        hsize_t n = 0;
+        H5Sget_simple_extent_dims(sp, &n, NULL);
+        H5Sclose(sp);
+        H5Dclose(d);
+        *n_records = (int64_t)n;
+        int i = 1;
+        for (;; ++i) {                    /* bh1, bh2, ... as read_trajectory */
+            char name[32];
+            snprintf(name, sizeof name, "bh%d", i);
+            if (H5Lexists(f, name, H5P_DEFAULT) <= 0) break;
+        }
+        *n_bh = i - 1;
+    }
+    H5Fclose(f);
+    return rc;
+}
+
+int bsn_read_trajectory(const char* filename, double* t, double* mass, double* rsink,
+                        double* pos, double* vel) {
+    int nb = 0;
+    int64_t n = 0;
+    int rc = bsn_trajectory_info(filename, &nb, &n);
+    if (rc != BSN_OK) return rc;
+    if (n > 0 && (!t || (nb > 0 && (!mass || !rsink || !pos || !vel))))
+        return fail(BSN_ERR_ARG, "NULL output");
+    if (n == 0) return BSN_OK;
+    quiet();
+    double* tmp = malloc(sizeof(double) * 3 * (size_t)n);
+    if (!tmp) return fail(BSN_ERR_ARG, "out of host memory");
+    hid_t f = H5Fopen(filename, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) { free(tmp); return fail(BSN_ERR_IO, "cannot open %s", filename); }
+    const hsize_t d1[1] = {(hsize_t)n}, d2[2] = {3, (hsize_t)n};
+    TRY(read_all(f, "time", 1, d1, t));
+    for (int i = 0; i < nb; ++i) {
+        char name[32];
+        snprintf(name, sizeof name, "bh%d", i + 1);
+        hid_t g = H5Gopen2(f, name, H5P_DEFAULT);
+        if (g < 0) { rc = fail(BSN_ERR_IO, "group %s missing", name); goto done; }
+        rc = read_all(g, "mass", 1, d1, mass + (size_t)i * n);
+        if (rc == BSN_OK) rc = read_all(g, "rsink", 1, d1, rsink + (size_t)i * n);
+        for (int k = 0; k < 2 && rc == BSN_OK; ++k) {
+            rc = read_all(g, k ? "vel" : "pos", 2, d2, tmp);
+            double* out = (k ? vel : pos) + (size_t)i * n * 3;
+            for (int64_t r = 0; r < n && rc == BSN_OK; ++r)
+                for (int a = 0; a < 3; ++a) out[3 * r + a] = tmp[(size_t)a * n + r];
+        }
+        H5Gclose(g);
+        if (rc != BSN_OK) goto done;
+    }
+done:
+    free(tmp);
+    H5Fclose(f);
+    return rc;
+}

@@ -2,8 +2,9 @@
 between a GPU-resident grid and disk through libbsn_io.so.
 
 Mirrors write_snapshot / read_snapshot (src/io.jl:29-84), save_checkpoint /
-load_checkpoint of the two-level FMR grid (src/checkpoint.jl:1-172) and
-save_checkpoint_uniform / load_checkpoint_uniform (src/checkpoint.jl:244-318).
+load_checkpoint of the two-level FMR grid (src/checkpoint.jl:1-172),
+save_checkpoint_uniform / load_checkpoint_uniform (src/checkpoint.jl:244-318)
+and the body trajectory file (src/io.jl:86-161).
 A Grid argument streams its state through pinned slabs (the device packs the
 file layout); a numpy array in the reference layout is written from the host.
 """
@@ -24,7 +25,8 @@ EXPORTS = [
     "bsn_io_last_error", "bsn_write_snapshot", "bsn_write_snapshot_host", "bsn_snapshot_info",
     "bsn_read_snapshot", "bsn_read_snapshot_host", "bsn_save_checkpoint_uniform",
     "bsn_checkpoint_uniform_info", "bsn_load_checkpoint_uniform", "bsn_save_checkpoint_fmr",
-    "bsn_checkpoint_fmr_info", "bsn_load_checkpoint_fmr",
+    "bsn_checkpoint_fmr_info", "bsn_load_checkpoint_fmr", "bsn_init_trajectory_file",
+    "bsn_append_trajectory", "bsn_trajectory_info", "bsn_read_trajectory",
 ]
 
 _io = None
@@ -35,6 +37,7 @@ GridParams = namedtuple("GridParams", "nx ny nz dx dy dz gamma x0 y0 z0 rho_floo
 CheckpointMeta = namedtuple("CheckpointMeta", "t step dt_last t_snap t_bh2_sink_on format_version")
 FMRCheckpointMeta = namedtuple("FMRCheckpointMeta", "t step dt_last format_version")
 FMRGridInfo = namedtuple("FMRGridInfo", "dims dxs box ratio gamma bc rho_floor P_floor")
+Trajectory = namedtuple("Trajectory", "mass rsink pos vel")
 
 
 def load_io():
@@ -64,6 +67,10 @@ def load_io():
                                      C.POINTER(_i)], _i),
         "bsn_load_checkpoint_fmr": ([_p, _p, _p, _p, _s, _pi64, _pd, _i, C.POINTER(_i), _p, _p,
                                      _p], _i),
+        "bsn_init_trajectory_file": ([_s, _i], _i),
+        "bsn_append_trajectory": ([_s, _d, _i, _p, _p, _p, _p], _i),
+        "bsn_trajectory_info": ([_s, C.POINTER(_i), _pi64], _i),
+        "bsn_read_trajectory": ([_s, _p, _p, _p, _p, _p], _i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -291,3 +298,34 @@ def load_checkpoint_host(path):
                      eps=float(props[i, 1]), c_code=float(props[i, 2]),
                      r_floor=float(props[i, 3])) for i in range(got.value)]
     return Uc, Uf, bhs, info, FMRCheckpointMeta(meta[0], step.value, meta[1], int(meta[2]))
+
+
+def init_trajectory_file(filename, n_bh: int) -> None:
+    """init_trajectory_file (src/io.jl:94-108)."""
+    _call("bsn_init_trajectory_file", _path(filename), int(n_bh))
+
+
+def append_trajectory(filename, t, bhs) -> None:
+    """append_trajectory (src/io.jl:115-133): one record of every body, with
+    its r_sink (src/BinarySupernova.jl:159)."""
+    from .sources import r_sink
+    n = len(bhs)
+    mass = np.ascontiguousarray([float(b.mass) for b in bhs], dtype=np.float64)
+    rs = np.ascontiguousarray([r_sink(b) for b in bhs], dtype=np.float64)
+    _, pos, vel, _ = _bodies(bhs)
+    _call("bsn_append_trajectory", _path(filename), float(t), n, _ptr(mass), _ptr(rs), _ptr(pos),
+          _ptr(vel))
+
+
+def read_trajectory(filename):
+    """read_trajectory (src/io.jl:144-161): (t, [Trajectory(mass, rsink,
+    pos (N x 3), vel (N x 3)) per body])."""
+    nb, n = C.c_int(), C.c_int64()
+    _call("bsn_trajectory_info", _path(filename), C.byref(nb), C.byref(n))
+    N, B = n.value, nb.value
+    t = np.zeros(N)
+    mass, rs = np.zeros((max(B, 1), N)), np.zeros((max(B, 1), N))
+    pos, vel = np.zeros((max(B, 1), N, 3)), np.zeros((max(B, 1), N, 3))
+    _call("bsn_read_trajectory", _path(filename), _ptr(t), _ptr(mass), _ptr(rs), _ptr(pos),
+          _ptr(vel))
+    return t, [Trajectory(mass[b], rs[b], pos[b], vel[b]) for b in range(B)]

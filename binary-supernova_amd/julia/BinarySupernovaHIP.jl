@@ -23,7 +23,8 @@ export HIPGrid, euler3d_step!, euler3d_rhs!, cfl_dt_3d, apply_floors_3d!,
        set_self_gravity!, gas_force_on_bh, accrete!, gas_diagnostics, gas_energy_total,
        gas_kinetic_total, gas_momentum_total, gas_angular_momentum_total, bound_gas_mass,
        write_snapshot, read_snapshot!, save_checkpoint_uniform, load_checkpoint_uniform!,
-       save_checkpoint, load_checkpoint!
+       save_checkpoint, load_checkpoint!, init_trajectory_file, append_trajectory,
+       read_trajectory
 
 const libbsn = joinpath(@__DIR__, "..", "libbsn_hip.so")
 const libbsn_io = joinpath(@__DIR__, "..", "libbsn_io.so")
@@ -473,6 +474,42 @@ function load_checkpoint!(G::HIPFMR, path::AbstractString; max_bodies::Int = 8)
               for i in 1:nb[]]
     return bodies, (; t = meta[1], step = step[], dt_last = meta[2],
                     format_version = Int(meta[3]))
+end
+
+"""init_trajectory_file(filename, n_bh) (src/io.jl:94-108)."""
+function init_trajectory_file(filename::String, n_bh::Int)
+    checkio(ccall((:bsn_init_trajectory_file, libbsn_io), Cint, (Cstring, Cint), filename,
+                  Cint(n_bh)))
+    return nothing
+end
+
+"""append_trajectory(filename, t, bhs, r_sink) (src/io.jl:115-133); r_sink is the
+reference's function (src/BinarySupernova.jl:159)."""
+function append_trajectory(filename::String, t::Real, bhs, r_sink)
+    checkio(ccall((:bsn_append_trajectory, libbsn_io), Cint,
+                  (Cstring, Float64, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                  filename, t, Cint(length(bhs)), Float64[bh.mass for bh in bhs],
+                  Float64[r_sink(bh) for bh in bhs], bh_pos(bhs), bh_vel(bhs)))
+    return nothing
+end
+
+"""read_trajectory(filename) (src/io.jl:144-161) -> (t, bh_data), bh_data one
+NamedTuple (mass, rsink, pos (N×3), vel (N×3)) per body."""
+function read_trajectory(filename::String)
+    nb = Ref{Cint}(0); n = Ref{Int64}(0)
+    checkio(ccall((:bsn_trajectory_info, libbsn_io), Cint, (Cstring, Ref{Cint}, Ref{Int64}),
+                  filename, nb, n))
+    N, B = Int(n[]), Int(nb[])
+    t = zeros(N); mass = zeros(max(B, 1) * N); rs = zeros(max(B, 1) * N)
+    pos = zeros(3 * max(B, 1) * N); vel = zeros(3 * max(B, 1) * N)
+    checkio(ccall((:bsn_read_trajectory, libbsn_io), Cint,
+                  (Cstring, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                  filename, t, mass, rs, pos, vel))
+    # the library writes body b, record r, axis a at (b*N + r)*3 + a
+    bh_data = [(mass = mass[b*N+1:(b+1)*N], rsink = rs[b*N+1:(b+1)*N],
+                pos = permutedims(reshape(pos[3b*N+1:3(b+1)*N], 3, N)),
+                vel = permutedims(reshape(vel[3b*N+1:3(b+1)*N], 3, N))) for b in 0:B-1]
+    return t, bh_data
 end
 
 end # module

@@ -90,6 +90,22 @@ int bsn_load_checkpoint_fmr(bsn_grid* coarse, bsn_grid* fine, double* Uc_host, d
                             const char* path, int64_t* step, double meta[3], int max_bodies,
                             int* n_bodies, double* pos, double* vel, double* props);
 
+/* init_trajectory_file(filename, n_bh) (src/io.jl:94-108): "time" and, per
+ * body, bh<i>/{mass, rsink, pos, vel}, extendable along the record axis. */
+int bsn_init_trajectory_file(const char* filename, int n_bh);
+/* append_trajectory(filename, t, bhs) (src/io.jl:115-133): one record for
+ * n_bh bodies; rsink is the caller's r_sink(bh) of each body, pos and vel 3
+ * values per body. */
+int bsn_append_trajectory(const char* filename, double t, int n_bh, const double* mass,
+                          const double* rsink, const double* pos, const double* vel);
+/* The body count and the number of records of a trajectory file. */
+int bsn_trajectory_info(const char* filename, int* n_bh, int64_t* n_records);
+/* read_trajectory(filename) (src/io.jl:144-161): t[N]; per body b,
+ * mass[b*N + r], rsink[b*N + r], pos[(b*N + r)*3 + a], vel likewise (the
+ * reference's N x 3 arrays, record by record). */
+int bsn_read_trajectory(const char* filename, double* t, double* mass, double* rsink,
+                        double* pos, double* vel);
+
 #ifdef __cplusplus
 }
 #endif

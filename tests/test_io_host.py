@@ -266,3 +266,44 @@ def test_fmr_and_uniform_checkpoints_are_not_cross_loadable(tmp_path):
     bad = dict(kw, box=(4, 6, 4, 5, 4, 5))
     with pytest.raises(bsn.BsnError, match="ratio"):
         bio.save_checkpoint_host(tmp_path / "bad.h5", Uc, Uf, bhs, **bad)
+
+
+def test_trajectory_file_round_trip_and_layout(tmp_path):
+    # init_trajectory_file / append_trajectory / read_trajectory
+    # (src/io.jl:94-161): three records of two bodies
+    B = bsn.BlackHole
+    p = str(tmp_path / "traj.h5")
+    bio.init_trajectory_file(p, 2)
+    recs = []
+    for k in range(3):
+        bhs = [B(pos=[0.1 * k, 0.2, -0.3], vel=[0.0, 0.5 + k, 0.0], mass=0.5 + 0.01 * k, eps=1e-3,
+                 c_code=20.0, r_floor=1e-3),
+               B(pos=[-0.1 * k, 0.0, 0.4], vel=[0.3, 0.0, -k], mass=0.25, eps=1e-3, c_code=20.0,
+                 r_floor=2e-2)]
+        bio.append_trajectory(p, 0.5 * k, bhs)
+        recs.append(bhs)
+    t, data = bio.read_trajectory(p)
+    np.testing.assert_array_equal(t, [0.0, 0.5, 1.0])
+    assert len(data) == 2
+    for i in range(2):
+        np.testing.assert_array_equal(data[i].mass, [r[i].mass for r in recs])
+        np.testing.assert_array_equal(data[i].rsink, [bsn.r_sink(r[i]) for r in recs])
+        np.testing.assert_array_equal(data[i].pos, [r[i].pos for r in recs])
+        np.testing.assert_array_equal(data[i].vel, [r[i].vel for r in recs])
+    # r_sink = max(6 M / c^2, r_floor): the floor of the second body wins
+    assert data[1].rsink[0] == 2e-2
+    h = header(p)
+    F = "H5T_IEEE_F64LE"
+    assert h["time"] == ("DATASET", F, (3,))
+    for i in (1, 2):
+        assert h[f"bh{i}/mass"] == ("DATASET", F, (3,))
+        assert h[f"bh{i}/rsink"] == ("DATASET", F, (3,))
+        # Julia's (N, 3) arrays appear as (3, N)
+        assert h[f"bh{i}/pos"] == ("DATASET", F, (3, 3))
+        np.testing.assert_array_equal(h5_array(p, f"/bh{i}/pos", (3, 3), tmp_path),
+                                      np.array([r[i - 1].pos for r in recs]).T)
+    assert "H5S_UNLIMITED" in h5dump("-H", p)
+    # an empty file reads back empty
+    bio.init_trajectory_file(str(tmp_path / "empty.h5"), 1)
+    t0, d0 = bio.read_trajectory(str(tmp_path / "empty.h5"))
+    assert len(t0) == 0 and len(d0) == 1 and len(d0[0].mass) == 0
