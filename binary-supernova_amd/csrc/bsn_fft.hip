@@ -253,7 +253,8 @@ __device__ __forceinline__ void reg_to_input(double2 (&x)[Q]) {
     for (int j = 0; j < Q; ++j) x[j] = t[j];
 }
 
-// Y forward, register form: block (m-range, k), 256 threads = (256/P columns) x P.
+// Y forward, register form: 256 threads = (256/P columns) x P; the columns of
+// a block are consecutive in the flat (k, m) order.
 template <int P, int Q>
 __global__ void __launch_bounds__(256) k_fftr_y_fwd(FftGeom F, const double2* __restrict__ A,
                                                     double2* __restrict__ B,
@@ -263,23 +264,26 @@ __global__ void __launch_bounds__(256) k_fftr_y_fwd(FftGeom F, const double2* __
     double2* tw = lds + N * TW;
     const int H = (F.N[0] >> 1) + 1, ny = F.n[1];
     const int c = threadIdx.x % TW, p = threadIdx.x / TW;
-    const int m = blockIdx.x * TW + c, k = blockIdx.y;
+    const int g = blockIdx.x * TW + c;                // flat column over (k, m)
+    const bool ok = g < H * F.n[2];
+    const int k = ok ? g / H : 0, m = ok ? g - k * H : 0;
     load_tw(tw, twy, N);
     double2 x[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = p + P * q;
-        x[q] = (q < Q / 2 && e < ny && m < H) ? A[((int64_t)k * ny + e) * H + m] : make_double2(0.0, 0.0);
+        x[q] = (q < Q / 2 && e < ny && ok) ? A[((int64_t)k * ny + e) * H + m] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_reg<P, Q, false>(x, p, c, TW, lds, tw);
-    if (m < H) {
+    if (ok) {
 #pragma unroll
         for (int j = 0; j < Q; ++j) B[((int64_t)k * N + reg_elem<P, Q>(p, j)) * H + m] = x[j];
     }
 }
 
-// Z, register form: forward, kernel spectrum, inverse; block (m-range, jj).
+// Z, register form: forward, kernel spectrum, inverse; columns in flat (jj, m)
+// order.
 template <int P, int Q>
 __global__ void __launch_bounds__(256) k_fftr_z(FftGeom F, double2* __restrict__ B,
                                                 const double* __restrict__ Kc,
@@ -289,18 +293,20 @@ __global__ void __launch_bounds__(256) k_fftr_z(FftGeom F, double2* __restrict__
     double2* tw = lds + N * TW;
     const int H = (F.N[0] >> 1) + 1, Ny = F.N[1], nz = F.n[2];
     const int c = threadIdx.x % TW, p = threadIdx.x / TW;
-    const int m = blockIdx.x * TW + c, jj = blockIdx.y;
+    const int g = blockIdx.x * TW + c;                // flat column over (jj, m)
+    const bool ok = g < H * Ny;
+    const int jj = ok ? g / H : 0, m = ok ? g - jj * H : 0;
     load_tw(tw, twz, N);
     double2 x[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = p + P * q;
-        x[q] = (q < Q / 2 && e < nz && m < H) ? B[((int64_t)e * Ny + jj) * H + m] : make_double2(0.0, 0.0);
+        x[q] = (q < Q / 2 && e < nz && ok) ? B[((int64_t)e * Ny + jj) * H + m] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_reg<P, Q, false>(x, p, c, TW, lds, tw);
     const int ky = min(jj, Ny - jj);
-    const int mm = m < H ? m : H - 1;
+    const int mm = m;
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
         const int e = reg_elem<P, Q>(p, j), kz = min(e, N - e);
@@ -309,7 +315,7 @@ __global__ void __launch_bounds__(256) k_fftr_z(FftGeom F, double2* __restrict__
     }
     reg_to_input<P, Q>(x);
     fft_reg<P, Q, true>(x, p, c, TW, lds, tw);
-    if (m < H) {
+    if (ok) {
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
             const int e = reg_elem<P, Q>(p, j);
@@ -318,7 +324,7 @@ __global__ void __launch_bounds__(256) k_fftr_z(FftGeom F, double2* __restrict__
     }
 }
 
-// Y inverse, register form: block (m-range, k); only j < ny kept.
+// Y inverse, register form: columns in flat (k, m) order; only j < ny kept.
 template <int P, int Q>
 __global__ void __launch_bounds__(256) k_fftr_y_inv(FftGeom F, const double2* __restrict__ B,
                                                     double2* __restrict__ A,
@@ -328,15 +334,17 @@ __global__ void __launch_bounds__(256) k_fftr_y_inv(FftGeom F, const double2* __
     double2* tw = lds + N * TW;
     const int H = (F.N[0] >> 1) + 1, ny = F.n[1];
     const int c = threadIdx.x % TW, p = threadIdx.x / TW;
-    const int m = blockIdx.x * TW + c, k = blockIdx.y;
+    const int g = blockIdx.x * TW + c;                // flat column over (k, m)
+    const bool ok = g < H * F.n[2];
+    const int k = ok ? g / H : 0, m = ok ? g - k * H : 0;
     load_tw(tw, twy, N);
     double2 x[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q)
-        x[q] = m < H ? B[((int64_t)k * N + p + P * q) * H + m] : make_double2(0.0, 0.0);
+        x[q] = ok ? B[((int64_t)k * N + p + P * q) * H + m] : make_double2(0.0, 0.0);
     __syncthreads();
     fft_reg<P, Q, true>(x, p, c, TW, lds, tw);
-    if (m < H) {
+    if (ok) {
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
             const int e = reg_elem<P, Q>(p, j);
@@ -636,7 +644,9 @@ static void launch_reg(int pass, hipStream_t s, const FftGeom& F, const RegArgs&
     constexpr int N = P * Q, TW = 256 / P;
     const int H = (F.N[0] >> 1) + 1, rows = F.n[1] * F.n[2];
     const size_t lds = (size_t)((N + Q) * TW + N / 2) * sizeof(double2);
-    const dim3 gy((H + TW - 1) / TW, pass == PASS_Z ? F.N[1] : F.n[2]);
+    // y and z passes: one flat index over (row, m), rows = nz (y) or Ny (z)
+    const int64_t cols = (int64_t)H * (pass == PASS_Z ? F.N[1] : F.n[2]);
+    const dim3 gy((unsigned)((cols + TW - 1) / TW));
     const dim3 gx((rows + TW - 1) / TW);
     switch (pass) {
         case PASS_X_FWD:
