@@ -12,7 +12,9 @@ from .euler3d import Grid, _cache, _ptr
 
 def solve_poisson_isolated(rho, nx, ny, nz, dx, dy, dz) -> np.ndarray:
     """Phi (same shape as rho, (nz, ny, nx) C-order = (nx, ny, nz) column-major)
-    of the active-cell density rho, computed on the GPU with hipFFT."""
+    of the active-cell density rho (solve_poisson_isolated,
+    src/self_gravity.jl:35-76), computed on the GPU: pruned register FFTs
+    when every zero-padded axis is a power of two, hipFFT otherwise."""
     rho = np.ascontiguousarray(rho, dtype=np.float64)
     if rho.shape != (nz, ny, nx):
         raise ValueError(f"rho shape {rho.shape} != {(nz, ny, nx)}")
