@@ -215,24 +215,35 @@ __device__ __forceinline__ double2 twN(const double2* tw, int m, int N) {
 // of one slot per P keeps the strided reads of the second step off a single
 // bank).
 template <int P, int Q, bool INV, bool ROWS = false>
-__device__ __forceinline__ void fft_reg(double2 (&x)[Q], int p, int c, int ncol, double2* lds,
+__device__ __forceinline__ void fft_reg(double2 (&x)[Q], int p, int c, int ncol, double* lds,
                                         const double2* tw) {
     constexpr int N = P * Q;
-    auto at = [&](int i) -> double2& {
+    auto at = [&](int i) -> double& {
         return ROWS ? lds[c * (N + Q) + i + i / P] : lds[i * ncol + c];
     };
     dft_reg<Q, INV>(x);
 #pragma unroll
     for (int s = 0; s < Q; ++s) x[s] = cmul(x[s], twN<INV>(tw, p * s, N));
+    // the exchange moves real parts, then imaginary parts, through one
+    // FP64 buffer: half the LDS of a complex buffer, so twice the blocks
+    // fit on a CU.  The real-part reads overwrite only .x, so .y still
+    // holds this thread's own values when they are stored.
     __syncthreads();                       // lds free (a previous transform read it)
 #pragma unroll
-    for (int s = 0; s < Q; ++s) at(s * P + p) = x[s];
+    for (int s = 0; s < Q; ++s) at(s * P + p) = x[s].x;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < Q / P; ++r)
+#pragma unroll
+        for (int k = 0; k < P; ++k) x[r * P + k].x = at((p + P * r) * P + k);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < Q; ++s) at(s * P + p) = x[s].y;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < Q / P; ++r) {
-        const int s = p + P * r;
 #pragma unroll
-        for (int k = 0; k < P; ++k) x[r * P + k] = at(s * P + k);
+        for (int k = 0; k < P; ++k) x[r * P + k].y = at((p + P * r) * P + k);
         dft_reg<P, INV>(x + r * P);
     }
 }
@@ -260,8 +271,8 @@ __global__ void __launch_bounds__(256) k_fftr_y_fwd(FftGeom F, const double2* __
                                                     double2* __restrict__ B,
                                                     const double2* __restrict__ twy) {
     constexpr int N = P * Q, TW = 256 / P;
-    extern __shared__ double2 lds[];                  // N * TW values, then N / 2 twiddles
-    double2* tw = lds + N * TW;
+    extern __shared__ double2 lds[];                  // N * TW FP64 values, then N / 2 twiddles
+    double2* tw = lds + N * TW / 2;
     const int H = (F.N[0] >> 1) + 1, ny = F.n[1];
     const int c = threadIdx.x % TW, p = threadIdx.x / TW;
     const int g = blockIdx.x * TW + c;                // flat column over (k, m)
@@ -275,7 +286,7 @@ __global__ void __launch_bounds__(256) k_fftr_y_fwd(FftGeom F, const double2* __
         x[q] = (q < Q / 2 && e < ny && ok) ? A[((int64_t)k * ny + e) * H + m] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_reg<P, Q, false>(x, p, c, TW, lds, tw);
+    fft_reg<P, Q, false>(x, p, c, TW, (double*)lds, tw);
     if (ok) {
 #pragma unroll
         for (int j = 0; j < Q; ++j) B[((int64_t)k * N + reg_elem<P, Q>(p, j)) * H + m] = x[j];
@@ -290,7 +301,7 @@ __global__ void __launch_bounds__(256) k_fftr_z(FftGeom F, double2* __restrict__
                                                 const double2* __restrict__ twz) {
     constexpr int N = P * Q, TW = 256 / P;
     extern __shared__ double2 lds[];
-    double2* tw = lds + N * TW;
+    double2* tw = lds + N * TW / 2;                  // after N * TW FP64 values
     const int H = (F.N[0] >> 1) + 1, Ny = F.N[1], nz = F.n[2];
     const int c = threadIdx.x % TW, p = threadIdx.x / TW;
     const int g = blockIdx.x * TW + c;                // flat column over (jj, m)
@@ -304,7 +315,7 @@ __global__ void __launch_bounds__(256) k_fftr_z(FftGeom F, double2* __restrict__
         x[q] = (q < Q / 2 && e < nz && ok) ? B[((int64_t)e * Ny + jj) * H + m] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_reg<P, Q, false>(x, p, c, TW, lds, tw);
+    fft_reg<P, Q, false>(x, p, c, TW, (double*)lds, tw);
     const int ky = min(jj, Ny - jj);
     const int mm = m;
 #pragma unroll
@@ -314,7 +325,7 @@ __global__ void __launch_bounds__(256) k_fftr_z(FftGeom F, double2* __restrict__
         x[j] = make_double2(x[j].x * kv, x[j].y * kv);
     }
     reg_to_input<P, Q>(x);
-    fft_reg<P, Q, true>(x, p, c, TW, lds, tw);
+    fft_reg<P, Q, true>(x, p, c, TW, (double*)lds, tw);
     if (ok) {
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
@@ -331,7 +342,7 @@ __global__ void __launch_bounds__(256) k_fftr_y_inv(FftGeom F, const double2* __
                                                     const double2* __restrict__ twy) {
     constexpr int N = P * Q, TW = 256 / P;
     extern __shared__ double2 lds[];
-    double2* tw = lds + N * TW;
+    double2* tw = lds + N * TW / 2;                  // after N * TW FP64 values
     const int H = (F.N[0] >> 1) + 1, ny = F.n[1];
     const int c = threadIdx.x % TW, p = threadIdx.x / TW;
     const int g = blockIdx.x * TW + c;                // flat column over (k, m)
@@ -343,7 +354,7 @@ __global__ void __launch_bounds__(256) k_fftr_y_inv(FftGeom F, const double2* __
     for (int q = 0; q < Q; ++q)
         x[q] = ok ? B[((int64_t)k * N + p + P * q) * H + m] : make_double2(0.0, 0.0);
     __syncthreads();
-    fft_reg<P, Q, true>(x, p, c, TW, lds, tw);
+    fft_reg<P, Q, true>(x, p, c, TW, (double*)lds, tw);
     if (ok) {
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
@@ -361,7 +372,7 @@ __global__ void __launch_bounds__(256) k_fftr_x_fwd(GridDev G, const double* __r
                                                     const double2* __restrict__ twx) {
     constexpr int N = P * Q, TR = 256 / P;
     extern __shared__ double2 lds[];
-    double2* tw = lds + (N + Q) * TR;
+    double2* tw = lds + (N + Q) * TR / 2;            // after (N + Q) * TR FP64 values
     const int H = (N >> 1) + 1, nx = F.n[0], rows = F.n[1] * F.n[2];
     const int p = threadIdx.x % P, c = threadIdx.x / P;
     const int row = blockIdx.x * TR + c;
@@ -384,7 +395,7 @@ __global__ void __launch_bounds__(256) k_fftr_x_fwd(GridDev G, const double* __r
         x[q] = make_double2(v, 0.0);
     }
     __syncthreads();
-    fft_reg<P, Q, false, true>(x, p, c, TR, lds, tw);
+    fft_reg<P, Q, false, true>(x, p, c, TR, (double*)lds, tw);
     if (row < rows) {
 #pragma unroll
         for (int jx = 0; jx < Q; ++jx) {
@@ -402,7 +413,7 @@ __global__ void __launch_bounds__(256) k_fftr_x_inv(FftGeom F, const double2* __
                                                     const double2* __restrict__ twx) {
     constexpr int N = P * Q, TR = 256 / P;
     extern __shared__ double2 lds[];
-    double2* tw = lds + (N + Q) * TR;
+    double2* tw = lds + (N + Q) * TR / 2;            // after (N + Q) * TR FP64 values
     const int H = (N >> 1) + 1, nx = F.n[0], rows = F.n[1] * F.n[2];
     const int p = threadIdx.x % P, c = threadIdx.x / P;
     const int row = blockIdx.x * TR + c;
@@ -420,7 +431,7 @@ __global__ void __launch_bounds__(256) k_fftr_x_inv(FftGeom F, const double2* __
         x[q] = v;
     }
     __syncthreads();
-    fft_reg<P, Q, true, true>(x, p, c, TR, lds, tw);
+    fft_reg<P, Q, true, true>(x, p, c, TR, (double*)lds, tw);
     if (row < rows) {
 #pragma unroll
         for (int jx = 0; jx < Q; ++jx) {
@@ -643,7 +654,7 @@ template <int P, int Q>
 static void launch_reg(int pass, hipStream_t s, const FftGeom& F, const RegArgs& a) {
     constexpr int N = P * Q, TW = 256 / P;
     const int H = (F.N[0] >> 1) + 1, rows = F.n[1] * F.n[2];
-    const size_t lds = (size_t)((N + Q) * TW + N / 2) * sizeof(double2);
+    const size_t lds = (size_t)(N + Q) * TW * sizeof(double) + (size_t)(N / 2) * sizeof(double2);
     // y and z passes: one flat index over (row, m), rows = nz (y) or Ny (z)
     const int64_t cols = (int64_t)H * (pass == PASS_Z ? F.N[1] : F.n[2]);
     const dim3 gy((unsigned)((cols + TW - 1) / TW));
