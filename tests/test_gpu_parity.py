@@ -714,8 +714,8 @@ def test_step_with_point_masses_sinks_and_phi_bitwise(form):
 
 
 def test_poisson_solve_matches_oracle_and_reference_checks():
-    # solve_poisson_isolated (src/self_gravity.jl:35-76) with hipFFT against the
-    # numpy restatement; FFT rounding differs, so to 1e-12 of max |phi|
+    # solve_poisson_isolated (src/self_gravity.jl:35-76) on the device against
+    # the numpy restatement; FFT rounding differs, so to 1e-12 of max |phi|
     from oracle import poisson as PO
     print("poisson", flush=True)
     rng = np.random.default_rng(81)
