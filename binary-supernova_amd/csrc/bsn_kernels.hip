@@ -476,10 +476,12 @@ template <bool FAST = false>
 __device__ __forceinline__ void cell_state(const GridDev& G, int64_t c, int stage, double (&u)[5],
                                            double (&un)[5]) {
     const bool need_u = stage > 1 || G.phi != nullptr || G.src != nullptr;
+    // each cell's state is read once per launch: streaming loads (the
+    // stand-alone update kernel 0.515 -> 0.544 of HBM peak at 256^3)
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-        un[q] = gmem(G.Un)[q * G.fs + c];
-        u[q] = need_u ? gmem(G.U)[q * G.fs + c] : 0.0;
+        un[q] = __builtin_nontemporal_load(&gmem(G.Un)[q * G.fs + c]);
+        u[q] = need_u ? __builtin_nontemporal_load(&gmem(G.U)[q * G.fs + c]) : 0.0;
     }
     if (need_u && (G.lmode & 2)) apply_floor<FAST>(u, G.rho_floor, G.P_floor, G.gamma);
 }
