@@ -272,7 +272,7 @@ def bench_turb(bsn, args):
     out["pcie_inclusive"] = host_array_rate(bsn, U, dims, d, gamma, r32["tune"])
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(U, dims, d, gamma, bc, floors, 1)
+        out["cpu_baseline"] = cpu_baseline(U, dims, d, gamma, bc, floors, 4)
     return out
 
 
@@ -383,7 +383,7 @@ def bench_uniform_pair(bsn, args, U, dims, d, gamma, bc, floors, workload, steps
                energy_drift_split=float(abs(tB[4] - t0[4]) / t0[4]),
                roofline=upd, roofline_flux=flux, tiles=r["tune"])
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(U, dims, d, gamma, bc, floors, 1)
+        out["cpu_baseline"] = cpu_baseline(U, dims, d, gamma, bc, floors, 30)
     return out
 
 
