@@ -13,6 +13,7 @@ Restated from:
   polytrope_ic_3d!        src/stellar_ic.jl:87-178
   bh_potential            src/gravity_bh.jl:28-31
   solve_poisson_isolated  src/self_gravity.jl:35-76   (numpy FFT, not FFTW)
+  restrict_density_to_coarse!  src/fmr_gravity.jl:40-60
   _prolong_scalar_trilinear!  src/fmr_gravity.jl:70-106
 """
 from __future__ import annotations
@@ -177,10 +178,19 @@ def polytrope_scales(gamma: float, R_star: float, rho_c: float):
 
 
 def polytrope_ic_3d(nx, ny, nz, dx, dy, dz, gamma, R_star=1.0, rho_c=1.0, x0=0.0, y0=0.0,
-                    z0=0.0, rho_floor=1e-10, P_floor=1e-12):
+                    z0=0.0, rho_floor=1e-10, P_floor=1e-12, M_star=None):
+    """polytrope_ic_3d! (src/stellar_ic.jl:87-178) without a hollow core:
+    rho = rho_c theta^n, P = K rho^(1+1/n) inside R_star, floors outside, at
+    rest.  The star is given by its central density rho_c, or, as in the
+    reference, by its mass M_star (then rho_c = M_star/(4 pi r_scale^3 omega_n)).
+    x0, y0, z0 are the left edge of the active domain; the star sits at the
+    origin."""
     n = 1.0 / (gamma - 1.0)
     xis, ths, _ = lane_emden(n)
     xi1 = xis[-1]
+    if M_star is not None:
+        omega = -xi1 ** 2 * lane_emden(n)[2][-1]
+        rho_c = float(M_star) / (4 * math.pi * (R_star / xi1) ** 3 * omega)
     M_star, r_scale, K, _ = polytrope_scales(gamma, R_star, rho_c)
     dxi = xis[1] - xis[0]
     xc = _centres(nx, dx, x0)[None, None, :]
@@ -236,6 +246,25 @@ def solve_poisson_isolated(rho: np.ndarray, dx: float, dy: float, dz: float) -> 
     pad[:nz, :ny, :nx] = rho
     phi = np.fft.irfftn(np.fft.rfftn(K) * np.fft.rfftn(pad), s=(Nz, Ny, Nx), axes=(0, 1, 2)) * dV
     return np.ascontiguousarray(phi[:nz, :ny, :nx])
+
+
+def restrict_density_to_coarse(rho_c: np.ndarray, rho_f: np.ndarray, box: tuple,
+                               ratio: int) -> None:
+    """restrict_density_to_coarse! (src/fmr_gravity.jl:40-60) on active-cell
+    density arrays (z, y, x): the coarse cells of box = (ci_lo, ci_hi, cj_lo,
+    cj_hi, ck_lo, ck_hi) (1-based) become the sum of their r^3 fine cells,
+    taken in the reference's order (x offset outermost, z innermost), times
+    1/r^3."""
+    ci_lo, ci_hi, cj_lo, cj_hi, ck_lo, ck_hi = (int(v) for v in box)
+    r = int(ratio)
+    n_i, n_j, n_k = ci_hi - ci_lo + 1, cj_hi - cj_lo + 1, ck_hi - ck_lo + 1
+    f = rho_f[:r * n_k, :r * n_j, :r * n_i]
+    val = np.zeros((n_k, n_j, n_i))
+    for sx in range(r):
+        for sy in range(r):
+            for sz in range(r):
+                val = val + f[sz::r, sy::r, sx::r]
+    rho_c[ck_lo - 1:ck_hi, cj_lo - 1:cj_hi, ci_lo - 1:ci_hi] = val * (1.0 / float(r ** 3))
 
 
 def prolong_scalar_trilinear(phi_c: np.ndarray, lo: tuple, ratio: int, nf: tuple) -> np.ndarray:
@@ -341,12 +370,7 @@ def fmr_star(base: int = 192, half: float = 8.0, gamma: float = 4.0 / 3.0, rho_f
         # composite density restricted to the base, solved once, prolonged down
         rho = [active(U)[..., 0].copy() for (U, _, _, _) in levels]
         for l in (2, 1):
-            bx = (b1, b2)[l - 1]
-            r = 2
-            f = rho[l]
-            nzc, nyc, nxc = f.shape[0] // r, f.shape[1] // r, f.shape[2] // r
-            avg = f.reshape(nzc, r, nyc, r, nxc, r).mean(axis=(1, 3, 5))
-            rho[l - 1][bx[4] - 1:bx[5], bx[2] - 1:bx[3], bx[0] - 1:bx[1]] = avg
+            restrict_density_to_coarse(rho[l - 1], rho[l], (b1, b2)[l - 1], 2)
         phi_c = solve_poisson_isolated(rho[0], dx0, dx0, dx0)
         phi_1 = prolong_scalar_trilinear(phi_c, (b1[0], b1[2], b1[4]), 2, lev[1][0])
         phi_2 = prolong_scalar_trilinear(phi_1, (b2[0], b2[2], b2[4]), 2, lev[2][0])
