@@ -41,9 +41,10 @@ class Batch:
 
     def set_update_shape(self, shape: int) -> None:
         """Fused z-sweep + update kernel shape 0..11 = 3 * variant +
-        segmentation: variant 0 plain, 1 operands loaded one cell ahead, 2 LDS
-        ring of the window's raw cells, 3 ring + early window load;
-        segmentation 0/1 = z runs of 16/32 cells, 2 = balanced run lengths."""
+        segmentation: variant 0 plain, 1 operands loaded one cell ahead (FP64)
+        or variant 3 without a register cap (split), 2 LDS ring of the
+        window's raw cells, 3 ring + early window load; segmentation 0/1 = z
+        runs of 16/32 cells, 2 = balanced run lengths."""
         call("bsn_batch_set_update_shape", self.handle, int(shape))
 
     def set_mode(self, inplace: bool) -> None:

@@ -146,6 +146,22 @@ static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int 
     int W, H;
     flux_shape_dims(dir, shape, W, H);
     std::vector<Tile> t;
+    if (dir != 0 && shape >= FLUX_WALK_SHAPE0) {
+        // register walks: 64 x 4 columns, runs of L interfaces along dir
+        const int L = flux_walk_len(shape);
+        for (int g = 0; g < (int)G.size(); ++g) {
+            const GridDev& d = G[g];
+            const int nf = (dir == 1 ? d.ny : d.nz) + 1;      // interfaces along dir
+            const int ncross = dir == 1 ? d.nz : d.ny;
+            for (int c0 = 0; c0 < ncross; c0 += 4)
+                for (int f0 = 0; f0 < nf; f0 += L) {
+                    const int len = nf - f0 < L ? nf - f0 : L;
+                    for (int i0 = 0; i0 < d.nx; i0 += 64)
+                        t.push_back({g, i0, c0, f0 | (len << ZT_LEN_SHIFT)});
+                }
+        }
+        return t;
+    }
     for (int g = 0; g < (int)G.size(); ++g) {
         const GridDev& d = G[g];
         if (dir == 0) {
@@ -854,6 +870,12 @@ extern "C" int bsn_grid_set_self_gravity(bsn_grid* g, int on) {
 
 extern "C" int bsn_grid_set_phi(bsn_grid* g, const double* phi, int where) {
     if (!g) return set_err(BSN_ERR_ARG, "NULL grid");
+    // a self-gravitating grid solves its own phi every stage into d_phi: a
+    // given phi would be overwritten, and freeing d_phi would leave the
+    // solve without a destination
+    if (g->self_grav)
+        return set_err(BSN_ERR_STATE, "self-gravity is on: phi is solved by the step "
+                                      "(bsn_grid_set_self_gravity(g, 0) first)");
     HIPCHK(hipSetDevice(g->ctx->device));
     HIPCHK(hipStreamSynchronize(g->ctx->stream));
     if (!phi) {
@@ -1359,6 +1381,26 @@ extern "C" int bsn_batch_set_mode(bsn_batch* b, int inplace) {
 
 // Times every tile shape of every kernel of a plan on its current states and
 // keeps the fastest; appends a JSON object of the timings to rep.
+// Name of a flux-sweep shape in the autotuner's report: "WxH" for a tiled
+// shape, "walkL" for a register walk of L interfaces.
+static std::string flux_shape_name(int dir, int shape) {
+    if (dir != 0 && shape >= FLUX_WALK_SHAPE0) return "walk" + std::to_string(flux_walk_len(shape));
+    int W, H;
+    flux_shape_dims(dir, shape, W, H);
+    return std::to_string(W) + "x" + std::to_string(H);
+}
+
+// Name of a fused-kernel shape in the autotuner's report: segmentation
+// (kz16, kz32, kbal) and variant.
+static const char* zshape_name(int prec, int shape) {
+    static const char* names[2][12] = {
+        {"kz16", "kz32", "kbal", "kz16pf", "kz32pf", "kbalpf", "kz16ring", "kz32ring", "kbalring",
+         "kz16ringpf", "kz32ringpf", "kbalringpf"},
+        {"kz16", "kz32", "kbal", "kz16ringpf2w", "kz32ringpf2w", "kbalringpf2w", "kz16ring",
+         "kz32ring", "kbalring", "kz16ringpf", "kz32ringpf", "kbalringpf"}};
+    return names[prec == 1][shape];
+}
+
 static int plan_autotune(Plan& P, std::string& out) {
     HIPCHK(hipSetDevice(P.ctx->device));
     CHK(plan_refresh(P));
@@ -1389,16 +1431,12 @@ static int plan_autotune(Plan& P, std::string& out) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             ms /= reps;
-            int W, H;
-            flux_shape_dims(d, s, W, H);
-            snprintf(buf, sizeof buf, "%s\"%dx%d\": %.4f", s ? ", " : "", W, H, ms);
+            snprintf(buf, sizeof buf, "%s\"%s\": %.4f", s ? ", " : "", flux_shape_name(d, s).c_str(), ms);
             rep += buf;
             if (ms < best_ms) { best_ms = ms; best = s; }
         }
         CHK(plan_set_shape(P, d, best));
-        int W, H;
-        flux_shape_dims(d, best, W, H);
-        snprintf(buf, sizeof buf, ", \"best\": \"%dx%d\"}", W, H);
+        snprintf(buf, sizeof buf, ", \"best\": \"%s\"}", flux_shape_name(d, best).c_str());
         rep += buf;
     }
     {
@@ -1419,16 +1457,12 @@ static int plan_autotune(Plan& P, std::string& out) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             ms /= reps;
-            static const char* var[4] = {"", "pf", "ring", "ringpf"};
-            static const char* sg[3] = {"kz16", "kz32", "kbal"};
-            snprintf(buf, sizeof buf, "%s\"%s%s\": %.4f", s ? ", " : "", sg[s % 3], var[s / 3], ms);
+            snprintf(buf, sizeof buf, "%s\"%s\": %.4f", s ? ", " : "", zshape_name(P.prec, s), ms);
             rep += buf;
             if (ms < best_ms) { best_ms = ms; best = s; }
         }
         CHK(plan_set_zshape(P, best));
-        static const char* vb[4] = {"", "pf", "ring", "ringpf"};
-        static const char* sb[3] = {"kz16", "kz32", "kbal"};
-        snprintf(buf, sizeof buf, ", \"best\": \"%s%s\"}", sb[best % 3], vb[best / 3]);
+        snprintf(buf, sizeof buf, ", \"best\": \"%s\"}", zshape_name(P.prec, best));
         rep += buf;
     }
     rep += "}";

@@ -265,6 +265,21 @@ template <> struct Prec<float> {
     }
 };
 
+// Stores the flux of interface f along DIR with cross indices ci, cj
+// (active, 0-based; ci runs along x for DIR 1 and 2, along y for DIR 0).
+// Streaming stores: the update reads each face flux once, from HBM (1.2% off
+// the split step at 256^3 against cached stores).
+template <int DIR, typename T>
+__device__ __forceinline__ void store_face(const GridDev& G, const T (&F)[5], int f, int ci, int cj) {
+    auto Fp = gmem((T*)G.F[DIR]);
+    int64_t o;
+    if (DIR == 0)      o = ((int64_t)cj * G.ny + ci) * G.fp[0] + f;
+    else if (DIR == 1) o = ((int64_t)cj * (G.ny + 1) + f) * G.fp[1] + ci;
+    else               o = ((int64_t)f * G.ny + cj) * G.fp[2] + ci;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) __builtin_nontemporal_store(F[q], &Fp[q * G.ffs[DIR] + o]);
+}
+
 template <int DIR, typename T, int W, int H>
 __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grids,
                                                  const Tile* __restrict__ tiles, int ntiles) {
@@ -280,15 +295,7 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
     auto emit = [&](const GridDev& G, const T (&c)[4][6], const T (&p)[6], int f, int ci, int cj) {
         T F[5];
         Prec<T>::template flux<DIR>(c, p, G, F);
-        auto Fp = gmem((T*)G.F[DIR]);
-        int64_t o;
-        if (DIR == 0)      o = ((int64_t)cj * G.ny + ci) * G.fp[0] + f;
-        else if (DIR == 1) o = ((int64_t)cj * (G.ny + 1) + f) * G.fp[1] + ci;
-        else               o = ((int64_t)f * G.ny + cj) * G.fp[2] + ci;
-        // streaming stores: the update reads each face flux once, from HBM
-        // (1.2% off the split step at 256^3 against cached stores)
-#pragma unroll
-        for (int q = 0; q < 5; ++q) __builtin_nontemporal_store(F[q], &Fp[q * G.ffs[DIR] + o]);
+        store_face<DIR>(G, F, f, ci, cj);
     };
     for (int t = t0; t < t1; ++t) {
         const Tile tl = tiles[t];
@@ -363,6 +370,65 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
             emit(G, c, p, f, ci, tl.c);
         }
         __syncthreads();
+    }
+}
+
+// y (DIR 1) and z (DIR 2) interface fluxes as register walks.  Tile
+// (g, i0, c0, f0 | len): 64 x 4 threads over (i, c) with c = k for DIR 1 and
+// c = j for DIR 2; each thread walks its column along DIR through the
+// interfaces f0 .. f0+len-1 with a six-cell register window, loading one new
+// cell per interface straight from HBM (lanes along x: coalesced), and the
+// split form carries the smoothness indicators that consecutive interfaces
+// share.  A run of len interfaces reads len + 5 cells.
+template <int DIR, typename T>
+__global__ __launch_bounds__(256) void k_fwalk(const GridDev* __restrict__ grids,
+                                               const Tile* __restrict__ tiles, int ntiles) {
+    constexpr bool FAST = sizeof(T) == 4;        // split form
+    int t0, t1;
+    chunk(ntiles, t0, t1);
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    for (int t = t0; t < t1; ++t) {
+        const Tile tl = tiles[t];
+        const GridDev& G = grids[tl.g];
+        const int f0 = tl.c & ZT_K0_MASK, len = tl.c >> ZT_LEN_SHIFT;
+        const int i = tl.a + lx, cc = tl.b + ly;
+        if (i >= G.nx || cc >= (DIR == 1 ? G.nz : G.ny)) continue;
+        T c[4][6], p[6];
+        double nxt[5];
+        // cells along the walk are read once (halo cells of neighbouring
+        // runs aside): streaming loads
+        auto fetch = [&](int pf) {
+            if (DIR == 1) load_cell<1, true, FAST, true>(G, NG + i, pf, NG + cc, nxt);
+            else          load_cell<2, true, FAST, true>(G, NG + i, NG + cc, pf, nxt);
+        };
+        auto place = [&](int slot) {
+            const double pr = Prec<T>::pressure(nxt, G.gamma);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) c[q][slot] = (T)nxt[q];
+            p[slot] = (T)pr;
+        };
+#pragma unroll
+        for (int m = 0; m < 5; ++m) { fetch(f0 + m); place(m); }
+        fetch(f0 + 5);
+        sp::WenoCarry wc[5];
+        if constexpr (FAST) sp::interface_walk_start(c, p, wc);
+        const int fend = f0 + len;
+        for (int f = f0; f < fend; ++f) {
+            place(5);                                   // cell f+5 (padded)
+            if (f + 1 < fend) fetch(f + 6);             // in flight during the flux
+            T F[5];
+            if constexpr (FAST)
+                sp::interface_flux<DIR, true>(c, p, G.gf32[0], G.gf32[1], G.gf32[2], F, wc);
+            else
+                Prec<T>::template flux<DIR>(c, p, G, F);
+            store_face<DIR>(G, F, f, i, cc);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) c[q][m] = c[q][m + 1];
+                p[m] = p[m + 1];
+            }
+        }
     }
 }
 
@@ -469,6 +535,16 @@ __device__ __forceinline__ void sink_add(const SrcDev& S, int i, int j, int k,
     }
 }
 
+// Flux divergence of one conserved field (src/gpu_kernels.jl:165-167).  The
+// split form's FP32 face fluxes are differenced in FP64 (exactly) and the
+// three terms summed with FMA; the FP64 form keeps the reference's operations.
+template <bool FAST>
+__device__ __forceinline__ double flux_div(double fx0, double fx1, double fy0, double fy1,
+                                           double fz0, double fz1, const GridDev& G) {
+    if (FAST) return -__builtin_fma(fx1 - fx0, G.idx, __builtin_fma(fy1 - fy0, G.idy, (fz1 - fz0) * G.idz));
+    return (-(fx1 - fx0)) * G.idx - (fy1 - fy0) * G.idy - (fz1 - fz0) * G.idz;
+}
+
 // Phi source, stage combination and floors of active cell (i, j, k) with the
 // flux divergence d (src/gpu_kernels.jl:291-319, src/euler3d.jl:244-257);
 // the result goes to Uout.  Returns the CFL speed bits of the new state.
@@ -520,9 +596,17 @@ __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, in
     double v[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-        if (stage == 1)      v[q] = un[q] + dt * d[q];
-        else if (stage == 2) v[q] = 0.75 * un[q] + 0.25 * u[q] + (0.25 * dt) * d[q];
-        else                 v[q] = (1.0 / 3.0) * un[q] + (2.0 / 3.0) * u[q] + ((2.0 / 3.0) * dt) * d[q];
+        // the split form combines with FMA (one rounding per term)
+        if (FAST) {
+            if (stage == 1)      v[q] = __builtin_fma(dt, d[q], un[q]);
+            else if (stage == 2) v[q] = __builtin_fma(0.25 * dt, d[q], __builtin_fma(0.25, u[q], 0.75 * un[q]));
+            else                 v[q] = __builtin_fma((2.0 / 3.0) * dt, d[q],
+                                                      __builtin_fma(2.0 / 3.0, u[q], (1.0 / 3.0) * un[q]));
+        } else {
+            if (stage == 1)      v[q] = un[q] + dt * d[q];
+            else if (stage == 2) v[q] = 0.75 * un[q] + 0.25 * u[q] + (0.25 * dt) * d[q];
+            else                 v[q] = (1.0 / 3.0) * un[q] + (2.0 / 3.0) * u[q] + ((2.0 / 3.0) * dt) * d[q];
+        }
     }
     apply_floor<FAST>(v, G.rho_floor, G.P_floor, G.gamma);
     // streaming stores: the new state is not read again by this kernel
@@ -575,7 +659,7 @@ __global__ __launch_bounds__(256) void k_update(const GridDev* __restrict__ grid
                     const double fy1 = (double)Fy[q * G.ffs[1] + oy + G.fp[1]];
                     fz1[q] = (double)Fz[q * G.ffs[2] + oz];
                     // src/gpu_kernels.jl:165-167
-                    d[q] = (-(fx1 - fx0)) * G.idx - (fy1 - fy0) * G.idy - (fz1[q] - fz0[q]) * G.idz;
+                    d[q] = flux_div<sizeof(T) == 4>(fx0, fx1, fy0, fy1, fz0[q], fz1[q], G);
                 }
                 if (stage == 0) {
 #pragma unroll
@@ -689,6 +773,10 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
 #pragma unroll
             for (int m = 0; m < 5; ++m) { fetch(k0 + m); place(m); }
             fetch(k0 + 5);
+            // split form: the smoothness indicators that consecutive
+            // interfaces share are carried along the column
+            sp::WenoCarry wc[5];
+            if constexpr (FAST) sp::interface_walk_start(c, p, wc);
             T fz0[5];                                   // the previous interface's z flux
             for (int f = k0; f <= kend; ++f) {
                 place(5);                               // cell f+5
@@ -696,7 +784,10 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
                 const int k = f - 1;                    // the cell finished in this iteration
                 if (PF == 1 && f > k0) operands(k);
                 T F[5];
-                Prec<T>::template flux<2>(c, p, G, F);
+                if constexpr (FAST)
+                    sp::interface_flux<2, true>(c, p, G.gf32[0], G.gf32[1], G.gf32[2], F, wc);
+                else
+                    Prec<T>::template flux<2>(c, p, G, F);
                 if (G.nzkeep > 0) {
                     // an interface the flux registers read: store it as the
                     // z-flux sweep would (adjacent tiles both compute their
@@ -716,9 +807,7 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
                     double d[5];
 #pragma unroll
                     for (int q = 0; q < 5; ++q)
-                        d[q] = (-((double)fx[1][q] - (double)fx[0][q])) * G.idx -
-                               ((double)fy[1][q] - (double)fy[0][q]) * G.idy -
-                               ((double)F[q] - (double)fz0[q]) * G.idz;
+                        d[q] = flux_div<FAST>(fx[0][q], fx[1][q], fy[0][q], fy[1][q], fz0[q], F[q], G);
                     const uint64_t b = finish_cell_with<FAST>(G, i, j, k, stage, d, u, un, do_cfl, dt);
                     smax = b > smax ? b : smax;
                 }
@@ -1312,8 +1401,21 @@ static hipError_t launch_flux_t(const LaunchCtx& L, const GridDev* grids, const 
 }
 
 template <int DIR, typename T>
+static hipError_t launch_fwalk_t(const LaunchCtx& L, const GridDev* grids, const Tile* tiles,
+                                 int ntiles, int max_wg) {
+    if (ntiles <= 0) return hipSuccess;
+    const void* fn = (const void*)&k_fwalk<DIR, T>;
+    int g = occupancy_grid(fn, 256, 0, ntiles, L.cus);
+    if (max_wg > 0 && g > max_wg) g = max_wg;
+    hipLaunchKernelGGL((k_fwalk<DIR, T>), dim3(g), dim3(256), 0, L.stream, grids, tiles, ntiles);
+    return hipGetLastError();
+}
+
+template <int DIR, typename T>
 static hipError_t launch_flux_shape(const LaunchCtx& L, int shape, const GridDev* grids,
                                     const Tile* tiles, int ntiles, int max_wg) {
+    if (DIR != 0 && shape >= FLUX_WALK_SHAPE0)
+        return launch_fwalk_t<DIR == 0 ? 1 : DIR, T>(L, grids, tiles, ntiles, max_wg);
     // shape ids: x sweep 0: 64x4, 1: 32x8, 2: 16x16;  y/z sweep 0: 64x4, 1: 64x8, 2: 32x8, 3: 64x2,
     // 4: 32x16
     if (DIR == 0) {
@@ -1333,14 +1435,23 @@ static hipError_t launch_flux_shape(const LaunchCtx& L, int shape, const GridDev
     }
 }
 
+// W x H of a tiled shape; for the walk shapes of the y and z sweeps W = 64
+// columns along x, H = 4 columns across, and the run length is
+// flux_walk_len(shape).
 void flux_shape_dims(int dir, int shape, int& W, int& H) {
     static const int xs[3][2] = {{64, 4}, {32, 8}, {16, 16}};
     static const int ys[5][2] = {{64, 4}, {64, 8}, {32, 8}, {64, 2}, {32, 16}};
+    if (dir != 0 && shape >= FLUX_WALK_SHAPE0 && shape < flux_shape_count(dir)) { W = 64; H = 4; return; }
     if (dir == 0) { if (shape < 0 || shape > 2) shape = 0; W = xs[shape][0]; H = xs[shape][1]; }
     else          { if (shape < 0 || shape > 4) shape = 0; W = ys[shape][0]; H = ys[shape][1]; }
 }
 
-int flux_shape_count(int dir) { return dir == 0 ? 3 : 5; }
+int flux_walk_len(int shape) {
+    static const int lens[3] = {16, 32, 64};
+    return lens[shape - FLUX_WALK_SHAPE0];
+}
+
+int flux_shape_count(int dir) { return dir == 0 ? 3 : FLUX_WALK_SHAPE0 + 3; }
 
 hipError_t launch_flux(const LaunchCtx& L, int dir, int prec, int shape, const GridDev* grids,
                        const Tile* tiles, int ntiles, int max_wg) {
@@ -1393,8 +1504,9 @@ static int zupdate_slots_t(int cus) {
 }
 
 // Shapes: 3 * variant + segmentation.  Variant 0 plain, 1 operands loaded one
-// cell ahead, 2 LDS ring of the window's raw cells, 3 ring plus an early load
-// of the next window cell.  Segmentation 0: 16 cells per column tile, 1: 32,
+// cell ahead (FP64) or variant 3 without a register cap (split: 2 waves per
+// SIMD), 2 LDS ring of the window's raw cells, 3 ring plus an early load of
+// the next window cell (split: both capped at 168 VGPRs, 3 waves per SIMD).  Segmentation 0: 16 cells per column tile, 1: 32,
 // 2: one length per grid chosen so the tile count fills the resident
 // workgroup slots (zupdate_slots) evenly.
 int zupdate_shape_count() { return 12; }
@@ -1411,9 +1523,9 @@ int zupdate_slots(int prec, int shape, int cus) {
         }
     }
     switch (v) {
-        case 1: return zupdate_slots_t<float, 1, 3, 0>(cus);
-        case 2: return zupdate_slots_t<float, 0, 1, 1>(cus);
-        case 3: return zupdate_slots_t<float, 2, 1, 1>(cus);
+        case 1: return zupdate_slots_t<float, 2, 1, 1>(cus);
+        case 2: return zupdate_slots_t<float, 0, 3, 1>(cus);
+        case 3: return zupdate_slots_t<float, 2, 3, 1>(cus);
         default: return zupdate_slots_t<float, 0, 1, 0>(cus);
     }
 }
@@ -1423,9 +1535,10 @@ static hipError_t launch_zupdate_p(const LaunchCtx& L, int shape, const GridDev*
                                    const Tile* tiles, int ntiles, int stage, int do_cfl) {
     constexpr bool F = sizeof(T) == 4;
     switch (shape / 3) {
-        case 1: return launch_zupdate_t<T, 1, F ? 3 : 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
-        case 2: return launch_zupdate_t<T, 0, 1, 1>(L, grids, tiles, ntiles, stage, do_cfl);
-        case 3: return launch_zupdate_t<T, 2, 1, 1>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 1: return F ? launch_zupdate_t<T, 2, 1, 1>(L, grids, tiles, ntiles, stage, do_cfl)
+                         : launch_zupdate_t<T, 1, 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 2: return launch_zupdate_t<T, 0, F ? 3 : 1, 1>(L, grids, tiles, ntiles, stage, do_cfl);
+        case 3: return launch_zupdate_t<T, 2, F ? 3 : 1, 1>(L, grids, tiles, ntiles, stage, do_cfl);
         default: return launch_zupdate_t<T, 0, 1, 0>(L, grids, tiles, ntiles, stage, do_cfl);
     }
 }

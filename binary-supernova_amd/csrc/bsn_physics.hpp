@@ -234,12 +234,13 @@ __device__ __forceinline__ void interface_flux(const double (&c)[4][6], const do
 
 // ---------------------------------------------------------------------------
 // Split precision.  FP32 with hardware reciprocal (v_rcp_f32, 1 ulp) and
-// square root (v_sqrt_f32); FMA contraction is allowed in this namespace.
+// square root (v_sqrt_f32).  Every fused multiply-add is written out
+// (__builtin_fma / __builtin_fmaf) and the compiler contracts nothing (the
+// library is built with -ffp-contract=off): an interface's flux then has the
+// same bits in every kernel that computes it, whatever code surrounds the
+// call, which the fused and in-place step forms and the tile and walk shapes
+// rely on.
 namespace sp {
-
-// A file-scope fp pragma applies to everything after it in the translation
-// unit, so contraction is switched on here and off again at the end of sp.
-#pragma clang fp contract(fast)
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -256,7 +257,8 @@ __device__ __forceinline__ double rcp64(double x) {
 __device__ __forceinline__ double cell_pressure(double r, double mx, double my, double mz, double E,
                                                 double g) {
     r = __builtin_fmax(r, 1e-30);
-    const double KE = 0.5 * (mx * mx + my * my + mz * mz) * rcp64(r);
+    const double m2 = __builtin_fma(mz, mz, __builtin_fma(my, my, mx * mx));
+    const double KE = (0.5 * m2) * rcp64(r);
     return __builtin_fmax((g - 1.0) * (E - KE), 0.0);
 }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -266,46 +268,136 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 __device__ __forceinline__ float fmaxj(float x, float y) { return __builtin_fmaxf(x, y); }
 __device__ __forceinline__ float fminj(float x, float y) { return __builtin_fminf(x, y); }
 
-// WENO5-Z weights.  eps is the reference's 1e-36 (a normal FP32 number),
-// raised to 1e-17*tau where needed so that (tau/(eps+beta))^2 stays below
-// 1e34 and cannot overflow FP32; whenever the raise is active the weight of
-// that stencil is already 1 - O(1e-34) in both forms.
-__device__ __forceinline__ float wenoz_combine(float b0, float b1, float b2, float d0, float d1,
-                                               float d2, float q0, float q1, float q2) {
-    float tau5 = fabsf(b0 - b2);
-    float eps = fmaxj(1e-36f, tau5 * 1e-17f);
-    float t0 = tau5 * rcp(eps + b0);
-    float t1 = tau5 * rcp(eps + b1);
-    float t2 = tau5 * rcp(eps + b2);
-    float a0 = d0 * (1.0f + t0 * t0);
-    float a1 = d1 * (1.0f + t1 * t1);
-    float a2 = d2 * (1.0f + t2 * t2);
-    float inv = rcp(a0 + a1 + a2);
-    return (a0 * q0 + a1 * q1 + a2 * q2) * inv;
+// WENO5-Z on three-cell positions.  The interface between a[2] and a[3]
+// reconstructs from four positions j = 0..3 of three consecutive cells
+// (x, y, z) = (a[j], a[j+1], a[j+2]).  Every smoothness indicator and
+// candidate polynomial of src/weno5.jl:25-61 is a function of one position's
+// second difference s = x - 2y + z and centred difference tB = x - z:
+//   t forms   tA = x - 4y + 3z = 2s - tB,   tC = 3x - 4y + z = 2s + tB,
+//   beta      13/12 s^2 + 1/4 t^2 = 13/12 (s^2 + 3/13 t^2),
+//   left  (d = 1/10, 6/10, 3/10): beta(A, j0), beta(B, j1), beta(C, j2),
+//                                 q = qA(j0), qB(j1), qR(j2)
+//   right (d = 3/10, 6/10, 1/10): beta(B, j1), beta(A, j2), beta(C, j3),
+//                                 q = qB(j1), qR(j2), qD(j3)
+// with qA = z + s/12 + tA/4 = 1/3 x - 7/6 y + 11/6 z,
+//      qB = y + s/12 - tB/4 = -1/6 x + 5/6 y + 1/3 z,
+//      qR = y - 5/12 s + tC/4 = 1/3 x + 5/6 y - 1/6 z,
+//      qD = x + s/12 + tC/4 = 11/6 x - 7/6 y + 1/3 z.
+// The WENO-Z weights are unchanged when every beta, tau and eps is scaled by
+// one factor, so the betas here are the reference's divided by 13/12.
+//
+// A constant reconstructs to itself exactly: s and t vanish, so every
+// candidate is the constant, and the combination is written as the middle
+// candidate plus weighted differences.  A uniform state at rest then has no
+// mass or energy flux in FP32 either (an outflow boundary does not leak).
+//
+// Each value is formed by explicit operations, so a value carried from one
+// interface to the next along a walk (the indicator or candidate of a
+// position the previous interfaces also used) has the bits a fresh evaluation
+// gives, and every kernel reconstructs an interface identically.
+
+struct Pos3 { float s, tB, ss; };
+
+__device__ __forceinline__ Pos3 pos3(float x, float y, float z) {
+    Pos3 P;
+    P.s = __builtin_fmaf(-2.0f, y, x + z);
+    P.tB = x - z;
+    P.ss = P.s * P.s;
+    return P;
+}
+__device__ __forceinline__ float tA(const Pos3& P) { return __builtin_fmaf(2.0f, P.s, -P.tB); }
+__device__ __forceinline__ float tC(const Pos3& P) { return __builtin_fmaf(2.0f, P.s, P.tB); }
+__device__ __forceinline__ float beta(const Pos3& P, float t) {
+    return __builtin_fmaf((3.0f / 13.0f) * t, t, P.ss);
+}
+__device__ __forceinline__ float qA(const Pos3& P, float ta, float z) {
+    return __builtin_fmaf(0.25f, ta, __builtin_fmaf(1.0f / 12.0f, P.s, z));
+}
+__device__ __forceinline__ float qB(const Pos3& P, float y) {
+    return __builtin_fmaf(-0.25f, P.tB, __builtin_fmaf(1.0f / 12.0f, P.s, y));
+}
+__device__ __forceinline__ float qR(const Pos3& P, float tc, float y) {
+    return __builtin_fmaf(0.25f, tc, __builtin_fmaf(-5.0f / 12.0f, P.s, y));
+}
+__device__ __forceinline__ float qD(const Pos3& P, float tc, float x) {
+    return __builtin_fmaf(0.25f, tc, __builtin_fmaf(1.0f / 12.0f, P.s, x));
 }
 
+// Left and right WENO5-Z (p = 2) values from the five indicators and four
+// candidate values of one interface.  One eps serves both sides: the
+// reference's 1e-36, raised to 1e-17 max(tau_L, tau_R) where needed so that
+// (tau / (eps + beta))^2 stays below 1e34 and cannot overflow FP32 (whenever
+// the raise is active the weight of that stencil is already 1 - O(1e-34));
+// the centre indicator's reciprocal is then shared.  The weights d_k are
+// scaled to (1, 6, 3) and (3, 6, 1); the normalisation removes the factor.
+__device__ __forceinline__ void wenoz_pair(float bA0, float bB1, float bC2, float bA2, float bC3,
+                                           float qa0, float qb1, float qr2, float qd3, float& vl,
+                                           float& vr) {
+    const float tauL = __builtin_fabsf(bA0 - bC2), tauR = __builtin_fabsf(bB1 - bC3);
+    const float eps = 1e-17f * __builtin_fmaxf(__builtin_fmaxf(1e-19f, tauL), tauR);
+    const float r1 = rcp(eps + bB1), r2 = rcp(eps + bA2);
+    {
+        const float t0 = tauL * rcp(eps + bA0), t1 = tauL * r1, t2 = tauL * rcp(eps + bC2);
+        const float u0 = __builtin_fmaf(t0, t0, 1.0f);
+        const float u1 = 6.0f * __builtin_fmaf(t1, t1, 1.0f);
+        const float u2 = 3.0f * __builtin_fmaf(t2, t2, 1.0f);
+        const float num = __builtin_fmaf(u2, qr2 - qb1, u0 * (qa0 - qb1));
+        vl = __builtin_fmaf(num, rcp((u0 + u1) + u2), qb1);
+    }
+    {
+        const float t0 = tauR * r1, t1 = tauR * r2, t2 = tauR * rcp(eps + bC3);
+        const float u0 = 3.0f * __builtin_fmaf(t0, t0, 1.0f);
+        const float u1 = 6.0f * __builtin_fmaf(t1, t1, 1.0f);
+        const float u2 = __builtin_fmaf(t2, t2, 1.0f);
+        const float num = __builtin_fmaf(u2, qd3 - qr2, u0 * (qb1 - qr2));
+        vr = __builtin_fmaf(num, rcp((u0 + u1) + u2), qr2);
+    }
+}
+
+// What a walk carries from one interface to the next, per quantity: beta(A)
+// and qA of the positions that become j = 0 and j = 1, and beta(C) of the one
+// that becomes j = 2 (the previous interface's j = 1, j = 2 and j = 3).
+struct WenoCarry { float A0, A1, C2, qA0, qA1; };
+
+// The carry before the first interface of a walk (positions 0..2 of that
+// interface, cells a[0..4]).
+__device__ __forceinline__ void weno5_walk_start(float a0, float a1, float a2, float a3, float a4,
+                                                 WenoCarry& w) {
+    const Pos3 P0 = pos3(a0, a1, a2), P1 = pos3(a1, a2, a3), P2 = pos3(a2, a3, a4);
+    const float ta0 = tA(P0), ta1 = tA(P1);
+    w.A0 = beta(P0, ta0);
+    w.qA0 = qA(P0, ta0, a2);
+    w.A1 = beta(P1, ta1);
+    w.qA1 = qA(P1, ta1, a3);
+    w.C2 = beta(P2, tC(P2));
+}
+
+// One interface: with WALK position 0 and the C indicator of position 2 come
+// from w (and w moves on to the next interface), otherwise all are formed here.
+template <bool WALK>
 __device__ __forceinline__ void weno5_pair(float a0, float a1, float a2, float a3, float a4,
-                                           float a5, float& vl, float& vr) {
-    const float c13 = 13.0f / 12.0f, c14 = 0.25f;
-    float qL0 = (1.0f / 3.0f) * a0 + (-7.0f / 6.0f) * a1 + (11.0f / 6.0f) * a2;
-    float qC = (-1.0f / 6.0f) * a1 + (5.0f / 6.0f) * a2 + (1.0f / 3.0f) * a3;
-    float qR = (1.0f / 3.0f) * a2 + (5.0f / 6.0f) * a3 + (-1.0f / 6.0f) * a4;
-    float qR2 = (11.0f / 6.0f) * a3 + (-7.0f / 6.0f) * a4 + (1.0f / 3.0f) * a5;
-    float s, t;
-    s = a0 - 2.0f * a1 + a2;  t = a0 - 4.0f * a1 + 3.0f * a2;
-    float bL0 = c13 * (s * s) + c14 * (t * t);
-    s = a1 - 2.0f * a2 + a3;  t = a1 - a3;
-    float bC = c13 * (s * s) + c14 * (t * t);
-    s = a2 - 2.0f * a3 + a4;
-    float h = c13 * (s * s);
-    t = 3.0f * a2 - 4.0f * a3 + a4;
-    float bL2 = h + c14 * (t * t);
-    t = a2 - 4.0f * a3 + 3.0f * a4;
-    float bR1 = h + c14 * (t * t);
-    s = a3 - 2.0f * a4 + a5;  t = 3.0f * a3 - 4.0f * a4 + a5;
-    float bR2 = c13 * (s * s) + c14 * (t * t);
-    vl = wenoz_combine(bL0, bC, bL2, 0.1f, 0.6f, 0.3f, qL0, qC, qR);
-    vr = wenoz_combine(bC, bR1, bR2, 0.3f, 0.6f, 0.1f, qC, qR, qR2);
+                                           float a5, WenoCarry& w, float& vl, float& vr) {
+    const Pos3 P1 = pos3(a1, a2, a3), P2 = pos3(a2, a3, a4), P3 = pos3(a3, a4, a5);
+    const float tc2 = tC(P2), tc3 = tC(P3), ta2 = tA(P2);
+    const float bB1 = beta(P1, P1.tB), bA2 = beta(P2, ta2), bC3 = beta(P3, tc3);
+    float bA0, bC2, qa0;
+    if (WALK) {
+        bA0 = w.A0;
+        qa0 = w.qA0;
+        bC2 = w.C2;
+        w.A0 = w.A1;
+        w.qA0 = w.qA1;
+        w.A1 = bA2;
+        w.qA1 = qA(P2, ta2, a4);
+        w.C2 = bC3;
+    } else {
+        const Pos3 P0 = pos3(a0, a1, a2);
+        const float ta0 = tA(P0);
+        bA0 = beta(P0, ta0);
+        qa0 = qA(P0, ta0, a2);
+        bC2 = beta(P2, tc2);
+    }
+    wenoz_pair(bA0, bB1, bC2, bA2, bC3, qa0, qB(P1, a2), qR(P2, tc2, a3), qD(P3, tc3, a3), vl, vr);
 }
 
 // The reference's HLLC (src/hllc.jl:17-165) in FP32: same branches and
@@ -326,14 +418,14 @@ __device__ __forceinline__ void hllc_normal(float rL, float mnL, float mt1L, flo
     if (m2L > (rL * rL) * 1e4f) {
         float s = (rL * 100.0f) * __builtin_amdgcn_rsqf(m2L);
         mnL *= s; mt1L *= s; mt2L *= s;
-        m2L = mnL * mnL + mt1L * mt1L + mt2L * mt2L;
-        KE_L = fminj(0.5f * m2L * irL, 0.5f * rL * 1e4f);
+        m2L = __builtin_fmaf(mt2L, mt2L, __builtin_fmaf(mt1L, mt1L, mnL * mnL));
+        KE_L = fminj((0.5f * m2L) * irL, rL * 5e3f);
     }
     if (m2R > (rR * rR) * 1e4f) {
         float s = (rR * 100.0f) * __builtin_amdgcn_rsqf(m2R);
         mnR *= s; mt1R *= s; mt2R *= s;
-        m2R = mnR * mnR + mt1R * mt1R + mt2R * mt2R;
-        KE_R = fminj(0.5f * m2R * irR, 0.5f * rR * 1e4f);
+        m2R = __builtin_fmaf(mt2R, mt2R, __builtin_fmaf(mt1R, mt1R, mnR * mnR));
+        KE_R = fminj((0.5f * m2R) * irR, rR * 5e3f);
     }
     EL = fmaxj(EL, KE_L);
     ER = fmaxj(ER, KE_R);
@@ -341,17 +433,18 @@ __device__ __forceinline__ void hllc_normal(float rL, float mnL, float mt1L, flo
     float vnR = mnR * irR;
     float pL = (g - 1.0f) * (EL - KE_L);
     float pR = (g - 1.0f) * (ER - KE_R);
-    float csL = fsqrt(g * pL * irL);
-    float csR = fsqrt(g * pR * irR);
+    float csL = fsqrt((g * pL) * irL);
+    float csR = fsqrt((g * pR) * irR);
     float SL, SR;
     if (pL > 0.0f && pR > 0.0f) {
         float r_avg = 0.5f * (rL + rR);
         float cs_avg = 0.5f * (csL + csR);
-        float p_star = fmaxj(0.0f, 0.5f * (pL + pR) - 0.5f * (vnR - vnL) * r_avg * cs_avg);
-        float qL = p_star > pL ? fsqrt(1.0f + gfac * (p_star * rcp(pL) - 1.0f)) : 1.0f;
-        float qR = p_star > pR ? fsqrt(1.0f + gfac * (p_star * rcp(pR) - 1.0f)) : 1.0f;
-        SL = vnL - csL * qL;
-        SR = vnR + csR * qR;
+        float b = (0.5f * (vnR - vnL)) * r_avg;
+        float p_star = fmaxj(0.0f, __builtin_fmaf(-b, cs_avg, 0.5f * (pL + pR)));
+        float qL = p_star > pL ? fsqrt(__builtin_fmaf(gfac, __builtin_fmaf(p_star, rcp(pL), -1.0f), 1.0f)) : 1.0f;
+        float qR = p_star > pR ? fsqrt(__builtin_fmaf(gfac, __builtin_fmaf(p_star, rcp(pR), -1.0f), 1.0f)) : 1.0f;
+        SL = __builtin_fmaf(-csL, qL, vnL);
+        SR = __builtin_fmaf(csR, qR, vnR);
     } else {
         SL = fminj(vnL - csL, vnR - csR);
         SR = fmaxj(vnL + csL, vnR + csR);
@@ -359,21 +452,21 @@ __device__ __forceinline__ void hllc_normal(float rL, float mnL, float mt1L, flo
     float dL = rL * (SL - vnL), dR = rR * (SR - vnR);
     float denom = dL - dR;
     if (fabsf(denom) < 1e-37f) { F[0] = F[1] = F[2] = F[3] = F[4] = 0.0f; return; }
-    float FmnL = rL * (vnL * vnL) + pL, FEL = (EL + pL) * vnL;
+    float FmnL = __builtin_fmaf(rL, vnL * vnL, pL), FEL = (EL + pL) * vnL;
     if (SL >= 0.0f) { F[0] = mnL; F[1] = FmnL; F[2] = mt1L * vnL; F[3] = mt2L * vnL; F[4] = FEL; return; }
-    float FmnR = rR * (vnR * vnR) + pR, FER = (ER + pR) * vnR;
+    float FmnR = __builtin_fmaf(rR, vnR * vnR, pR), FER = (ER + pR) * vnR;
     if (SR <= 0.0f) { F[0] = mnR; F[1] = FmnR; F[2] = mt1R * vnR; F[3] = mt2R * vnR; F[4] = FER; return; }
     if (fmaxj(rL, rR) > 3.0f * fminj(rL, rR)) {
         float inv_dS = rcp(SR - SL);
         float SLSR = SL * SR;
-        F[0] = (SR * mnL - SL * mnR + SLSR * (rR - rL)) * inv_dS;
-        F[1] = (SR * FmnL - SL * FmnR + SLSR * (mnR - mnL)) * inv_dS;
-        F[2] = (SR * (mt1L * vnL) - SL * (mt1R * vnR) + SLSR * (mt1R - mt1L)) * inv_dS;
-        F[3] = (SR * (mt2L * vnL) - SL * (mt2R * vnR) + SLSR * (mt2R - mt2L)) * inv_dS;
-        F[4] = (SR * FEL - SL * FER + SLSR * (ER - EL)) * inv_dS;
+        F[0] = __builtin_fmaf(SLSR, rR - rL, __builtin_fmaf(SR, mnL, -(SL * mnR))) * inv_dS;
+        F[1] = __builtin_fmaf(SLSR, mnR - mnL, __builtin_fmaf(SR, FmnL, -(SL * FmnR))) * inv_dS;
+        F[2] = __builtin_fmaf(SLSR, mt1R - mt1L, __builtin_fmaf(SR, mt1L * vnL, -(SL * (mt1R * vnR)))) * inv_dS;
+        F[3] = __builtin_fmaf(SLSR, mt2R - mt2L, __builtin_fmaf(SR, mt2L * vnL, -(SL * (mt2R * vnR)))) * inv_dS;
+        F[4] = __builtin_fmaf(SLSR, ER - EL, __builtin_fmaf(SR, FEL, -(SL * FER))) * inv_dS;
         return;
     }
-    float S_star = ((pR - pL) + mnL * (SL - vnL) - mnR * (SR - vnR)) * rcp(denom);
+    float S_star = __builtin_fmaf(-mnR, SR - vnR, __builtin_fmaf(mnL, SL - vnL, pR - pL)) * rcp(denom);
     // Star-state jumps written through del = (S* - vn) / (S - S*), so that
     // chi = rho (1 + del) and every jump U* - U vanishes exactly when S* = vn
     // (identical states at rest give zero mass and energy flux in FP32 too);
@@ -386,36 +479,42 @@ __device__ __forceinline__ void hllc_normal(float rL, float mnL, float mt1L, flo
     const float vn = lft ? vnL : vnR, S = lft ? SL : SR, d = lft ? dL : dR;
     const float Fmn = lft ? FmnL : FmnR, FE = lft ? FEL : FER;
     float del = (S_star - vn) * rcp(S - S_star);
-    float chi = r + r * del;
-    float dE = fabsf(d) > 1e-37f ? E * del + chi * (S_star - vn) * (S_star + p * rcp(d)) : -E;
-    F[0] = mn + S * (r * del);
-    F[1] = Fmn + S * (chi * S_star - mn);
-    F[2] = mt1 * vn + S * (mt1 * del);
-    F[3] = mt2 * vn + S * (mt2 * del);
-    F[4] = FE + S * dE;
+    float chi = __builtin_fmaf(r, del, r);
+    float dE = fabsf(d) > 1e-37f
+                   ? __builtin_fmaf(E, del, (chi * (S_star - vn)) * __builtin_fmaf(p, rcp(d), S_star))
+                   : -E;
+    F[0] = __builtin_fmaf(S, r * del, mn);
+    F[1] = __builtin_fmaf(S, __builtin_fmaf(chi, S_star, -mn), Fmn);
+    F[2] = __builtin_fmaf(S, mt1 * del, mt1 * vn);
+    F[3] = __builtin_fmaf(S, mt2 * del, mt2 * vn);
+    F[4] = __builtin_fmaf(S, dE, FE);
 }
 
-template <int DIR>
+// Interface flux along DIR from the six-cell stencil c[q][s], p[s] (as in
+// the FP64 form).  WALK: the indicators of positions 0 and 2 come from the
+// carry w[5] of the quantities (rho, m_x, m_y, m_z, P), which moves on.
+template <int DIR, bool WALK = false>
 __device__ __forceinline__ void interface_flux(const float (&c)[4][6], const float (&p)[6],
-                                               float g, float gfac, float ig1, float F[5]) {
+                                               float g, float gfac, float ig1, float F[5],
+                                               WenoCarry (&w)[5]) {
     float L[4], R[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-        weno5_pair(c[q][0], c[q][1], c[q][2], c[q][3], c[q][4], c[q][5], L[q], R[q]);
+        weno5_pair<WALK>(c[q][0], c[q][1], c[q][2], c[q][3], c[q][4], c[q][5], w[q], L[q], R[q]);
     float P_maxL = fmaxj(fmaxj(fmaxj(fmaxj(p[0], p[1]), p[2]), p[3]), p[4]);
     float P_maxR = fmaxj(fmaxj(fmaxj(fmaxj(p[1], p[2]), p[3]), p[4]), p[5]);
     float PLw, PRw;
-    weno5_pair(p[0], p[1], p[2], p[3], p[4], p[5], PLw, PRw);
+    weno5_pair<WALK>(p[0], p[1], p[2], p[3], p[4], p[5], w[4], PLw, PRw);
     float PL = fminj(fmaxj(PLw, 0.0f), P_maxL);
     float PR = fminj(fmaxj(PRw, 0.0f), P_maxR);
     float rpL = fmaxj(L[0], 1e-30f), rpR = fmaxj(R[0], 1e-30f);
     float irL = rcp(rpL), irR = rcp(rpR);
-    float m2L = L[1] * L[1] + L[2] * L[2] + L[3] * L[3];
-    float m2R = R[1] * R[1] + R[2] * R[2] + R[3] * R[3];
-    float KEL = fminj(0.5f * m2L * irL, 0.5f * rpL * 1e4f);
-    float KER = fminj(0.5f * m2R * irR, 0.5f * rpR * 1e4f);
-    float EL = PL * ig1 + KEL;
-    float ER = PR * ig1 + KER;
+    float m2L = __builtin_fmaf(L[3], L[3], __builtin_fmaf(L[2], L[2], L[1] * L[1]));
+    float m2R = __builtin_fmaf(R[3], R[3], __builtin_fmaf(R[2], R[2], R[1] * R[1]));
+    float KEL = fminj((0.5f * m2L) * irL, rpL * 5e3f);
+    float KER = fminj((0.5f * m2R) * irR, rpR * 5e3f);
+    float EL = __builtin_fmaf(PL, ig1, KEL);
+    float ER = __builtin_fmaf(PR, ig1, KER);
     float G[5];
     if (DIR == 0) {
         hllc_normal(L[0], L[1], L[2], L[3], EL, m2L, irL, KEL, R[0], R[1], R[2], R[3], ER, m2R, irR, KER,
@@ -432,7 +531,21 @@ __device__ __forceinline__ void interface_flux(const float (&c)[4][6], const flo
     }
 }
 
-#pragma clang fp contract(off)
+template <int DIR>
+__device__ __forceinline__ void interface_flux(const float (&c)[4][6], const float (&p)[6],
+                                               float g, float gfac, float ig1, float F[5]) {
+    WenoCarry w[5];
+    interface_flux<DIR, false>(c, p, g, gfac, ig1, F, w);
+}
+
+// Starts a walk along DIR: the carry for the first interface from cells
+// 0..4 of its stencil.
+__device__ __forceinline__ void interface_walk_start(const float (&c)[4][6], const float (&p)[6],
+                                                     WenoCarry (&w)[5]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) weno5_walk_start(c[q][0], c[q][1], c[q][2], c[q][3], c[q][4], w[q]);
+    weno5_walk_start(p[0], p[1], p[2], p[3], p[4], w[4]);
+}
 
 }  // namespace sp
 }  // namespace bsn

@@ -225,11 +225,42 @@ def cfl_dt_3d(U, nx, ny, nz, dx, dy, dz, gamma, cfl) -> float:
     return out.value
 
 
+def _check_state_array(a, g: Grid, name: str) -> None:
+    if (not isinstance(a, np.ndarray) or a.shape != g.shape or a.dtype != np.float64
+            or not a.flags.c_contiguous):
+        raise ValueError(f"{name} must be a C-contiguous float64 array of shape {g.shape}")
+
+
+_sg_cache: dict = {}
+
+
 def _apply_sg_force_level_(dU, U, phi, nx, ny, nz, dx, dy, dz):
-    """_apply_sg_force_level! (src/fmr_gravity.jl:146-156) on a host dU."""
-    g, _ = _as_grid(U, nx, ny, nz)
+    """_apply_sg_force_level! (src/fmr_gravity.jl:146-156) on a host dU.
+
+    A numpy U goes through a grid of its own (not the per-shape grid the step
+    functions share), so the phi set here never reaches a later step; a Grid
+    U gets its previous phi (or none) back before returning."""
+    if isinstance(U, Grid):
+        g = U
+        if (g.nx, g.ny, g.nz) != (nx, ny, nz):
+            raise ValueError("grid dims do not match nx, ny, nz")
+        own, prev = False, g._phi
+    else:
+        if not isinstance(U, np.ndarray) or U.dtype != np.float64 or not U.flags.c_contiguous:
+            raise TypeError("U must be a Grid or a C-contiguous float64 numpy array")
+        key = (nx, ny, nz)
+        g = _sg_cache.get(key)
+        if g is None:
+            g = _sg_cache[key] = Grid(nx, ny, nz)
+        g.set_state(U)
+        own, prev = True, None
+    _check_state_array(dU, g, "dU")
     g.set_phi(phi)
-    call("bsn_apply_sg_force", g.handle, _ptr(dU), BSN_HOST, float(dx), float(dy), float(dz))
+    try:
+        call("bsn_apply_sg_force", g.handle, _ptr(dU), BSN_HOST, float(dx), float(dy), float(dz))
+    finally:
+        if not own:
+            g.set_phi(prev)
 
 
 def rk3_step_(U: Grid, L_, dt, tmp1, tmp2=None):
@@ -237,6 +268,7 @@ def rk3_step_(U: Grid, L_, dt, tmp1, tmp2=None):
     host array tmp1 with L(U); the base state Un lives on the device."""
     if not isinstance(U, Grid):
         raise TypeError("rk3_step_ advances a Grid")
+    _check_state_array(tmp1, U, "tmp1")
     call("bsn_rk3_save", U.handle)
     for stage in (1, 2, 3):
         L_(tmp1, U)

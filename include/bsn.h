@@ -68,7 +68,8 @@ int bsn_grid_destroy(bsn_grid* g);
 int bsn_grid_set_state(bsn_grid* g, const double* U, int where);
 int bsn_grid_get_state(bsn_grid* g, double* U, int where);
 /* phi for the gravity source (_apply_sg_force_level!, src/fmr_gravity.jl:146);
- * NULL removes it.  The grid keeps its own device copy. */
+ * NULL removes it.  The grid keeps its own device copy.  BSN_ERR_STATE while
+ * self-gravity is on (the step solves phi itself). */
 int bsn_grid_set_phi(bsn_grid* g, const double* phi, int where);
 /* Geometry and physics of the grid, used by the step functions below. */
 int bsn_grid_set_params(bsn_grid* g, double dx, double dy, double dz, double gamma, int bc,
@@ -224,7 +225,8 @@ int bsn_batch_destroy(bsn_batch* b);
  * the shapes the autotuner keeps for a 256^3 box: split x 0, y/z 2, fused
  * kernel 11; FP64 x 2, y/z 2, fused kernel 2. */
 int bsn_batch_autotune(bsn_batch* b, char* report, int report_len);
-/* Sets the flux tile shape ids explicitly (x: 0..2, y/z: 0..4). */
+/* Sets the flux shape ids explicitly (x: 0..2 tiled; y/z: 0..4 tiled, 5..7
+ * register walks along the sweep of 16, 32 or 64 interfaces per thread). */
 int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz);
 /* Sets the shape id (0..11) of the fused z-sweep + stage-update kernel used
  * by the out-of-place step: shape = 3 * variant + segmentation.  Variant 0

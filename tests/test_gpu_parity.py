@@ -96,6 +96,37 @@ def test_apply_sg_force_bitwise():
     np.testing.assert_array_equal(d2, d1)
 
 
+def test_apply_sg_force_leaves_no_phi_on_later_steps():
+    # a step of the same shape after _apply_sg_force_level_ has no phi source
+    nx, ny, nz = 8, 8, 8
+    print("sg force then step (8, 8, 8)", flush=True)
+    U = random_state(nx, ny, nz, seed=17, wild=False)
+    phi = np.random.default_rng(4).normal(size=(nz, ny, nx))
+    bsn._apply_sg_force_level_(np.zeros(U.shape), U.copy(), phi, nx, ny, nz, 0.1, 0.1, 0.1)
+    Ug = U.copy()
+    O.euler3d_step(U, 0.1, 0.07, 0.13, 1e-3, 1.4, "outflow", 1e-4, 1e-4)
+    bsn.euler3d_step_(Ug, nx, ny, nz, 0.1, 0.07, 0.13, 1e-3, 1.4, rho_floor=1e-4, P_floor=1e-4)
+    np.testing.assert_array_equal(Ug, U)
+    # a wrong-shaped dU is refused before any device work
+    with pytest.raises(ValueError):
+        bsn._apply_sg_force_level_(np.zeros((3, 3)), U.copy(), phi, nx, ny, nz, 0.1, 0.1, 0.1)
+
+
+def test_set_phi_refused_while_self_gravity_is_on():
+    # phi belongs to the step's own solve: setting or removing it is an error
+    nx, ny, nz = 16, 8, 16
+    print("set_phi on a self-gravitating grid", flush=True)
+    g = bsn.Grid(nx, ny, nz).set_state(_gravitating_blob(nx, ny, nz, 0.1))
+    g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
+    bsn.set_self_gravity(g, True)
+    for phi in (None, np.zeros((nz, ny, nx))):
+        with pytest.raises(bsn.BsnError, match="self-gravity is on"):
+            g.set_phi(phi)
+    bsn.euler3d_step_(g, nx, ny, nz, 0.1, 0.1, 0.1, 1e-3, 1.4, rho_floor=1e-4, P_floor=1e-4)
+    bsn.set_self_gravity(g, False)
+    g.set_phi(None)
+
+
 @pytest.mark.parametrize("shape", [(8, 8, 8), (33, 17, 65)])
 def test_cfl_bitwise(shape):
     nx, ny, nz = shape
@@ -379,6 +410,50 @@ def test_batch_matches_single_bitwise():
         np.testing.assert_array_equal(g.get_state(), U)
 
 
+def test_batch_of_48_uneven_problems():
+    # The batch workload itself: 48 Sod / Sedov problems of uneven sizes
+    # (batch_problems(48, seed=4321), 18.2 M cells) in one Batch, autotuned,
+    # two device-dt steps captured as hipGraphs.  FP64: every problem equals
+    # the oracle's host loop (cfl_dt_3d, then euler3d_step!) bit for bit.
+    # Split: L1(rho) against FP64 <= 1e-5 and the mass within 1e-12 of FP64's.
+    print("batch of 48 uneven problems", flush=True)
+    probs = P.batch_problems(48, seed=4321)
+    nsteps = 2
+
+    def run(prec):
+        grids = []
+        for p in probs:
+            g = bsn.Grid(*p["dims"], prec).set_state(p["U"])
+            g.set_params(*p["d"], p["gamma"], p["bc"], *p["floors"])
+            grids.append(g)
+        b = bsn.Batch(grids)
+        b.autotune()
+        b.run(0.4, nsteps, graph=True)
+        t, _ = b.times()
+        return [g.get_state() for g in grids], t
+
+    A, tA = run("fp64")
+    for n, (p, S) in enumerate(zip(probs, A)):
+        V = p["U"].copy()
+        t = 0.0
+        for _ in range(nsteps):
+            dt = O.cfl_dt_3d(V, *p["d"], p["gamma"], 0.4)
+            O.euler3d_step(V, *p["d"], dt, p["gamma"], p["bc"], *p["floors"])
+            t += dt
+        assert tA[n] == t, (n, p["kind"], p["dims"])
+        np.testing.assert_array_equal(S, V, err_msg=f"problem {n} {p['kind']} {p['dims']}")
+    B, _ = run("split")
+    l1n = l1d = 0.0
+    for p, a, b_ in zip(probs, A, B):
+        ra, rb = a[NG:-NG, NG:-NG, NG:-NG, 0], b_[NG:-NG, NG:-NG, NG:-NG, 0]
+        l1n += float(np.abs(ra - rb).sum())
+        l1d += float(np.abs(ra).sum())
+        dV = p["d"][0] * p["d"][1] * p["d"][2]
+        ma, mb = ra.sum() * dV, rb.sum() * dV
+        assert abs(mb - ma) <= 1e-12 * ma, (p["kind"], p["dims"], ma, mb)
+    assert l1n / l1d <= 1e-5, l1n / l1d
+
+
 def test_device_dt_run_graph_equals_host_loop():
     print("graph", flush=True)
     shapes = [(24, 20, 16), (16, 16, 40)]
@@ -416,9 +491,11 @@ def test_every_tile_shape_gives_the_same_bits(prec):
     s = (40, 36, 20)
     U = random_state(*s, seed=21)
     ref = None
-    # x shapes 0..2, y/z shapes 0..4; the z shape acts in the in-place form
+    # x shapes 0..2, y/z shapes 0..4 tiled and 5..7 register walks of 16, 32
+    # and 64 interfaces (runs end ragged on 37 and 21 interfaces); the z shape
+    # acts in the in-place form
     for sx in range(3):
-        for sy in range(5):
+        for sy in range(8):
             for inplace in (False, True):
                 g = bsn.Grid(*s, prec).set_state(U)
                 g.set_params(0.1, 0.1, 0.1, 1.4, "outflow", 1e-4, 1e-4)
