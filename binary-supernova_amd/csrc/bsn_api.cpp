@@ -146,9 +146,26 @@ static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int 
     int W, H;
     flux_shape_dims(dir, shape, W, H);
     std::vector<Tile> t;
-    if (dir != 0 && shape >= FLUX_WALK_SHAPE0) {
+    if (dir == 0 && flux_walk_len(0, shape) > 0) {
+        // x walks: 4 rows per tile, interfaces 0 .. nx-1 of a row in full
+        // tiles of 64 K (K per lane) and one for the rest; edge tiles take
+        // the last interface of 256 rows each
+        const int per = 64 * flux_walk_len(0, shape);
+        for (int g = 0; g < (int)G.size(); ++g) {
+            const GridDev& d = G[g];
+            for (int k = 0; k < d.nz; ++k)
+                for (int j0 = 0; j0 < d.ny; j0 += 4)
+                    for (int a = 0; a < d.nx; a += per) {
+                        const int n = d.nx - a < per ? d.nx - a : per;
+                        t.push_back({g, a, j0, k | (n << ZT_LEN_SHIFT)});
+                    }
+            for (int r0 = 0; r0 < d.ny * d.nz; r0 += 256) t.push_back({g, -1, r0, 0});
+        }
+        return t;
+    }
+    if (dir != 0 && flux_walk_len(dir, shape) > 0) {
         // register walks: 64 x 4 columns, runs of L interfaces along dir
-        const int L = flux_walk_len(shape);
+        const int L = flux_walk_len(dir, shape);
         for (int g = 0; g < (int)G.size(); ++g) {
             const GridDev& d = G[g];
             const int nf = (dir == 1 ? d.ny : d.nz) + 1;      // interfaces along dir
@@ -1384,7 +1401,9 @@ extern "C" int bsn_batch_set_mode(bsn_batch* b, int inplace) {
 // Name of a flux-sweep shape in the autotuner's report: "WxH" for a tiled
 // shape, "walkL" for a register walk of L interfaces.
 static std::string flux_shape_name(int dir, int shape) {
-    if (dir != 0 && shape >= FLUX_WALK_SHAPE0) return "walk" + std::to_string(flux_walk_len(shape));
+    if (flux_walk_len(dir, shape) > 0)
+        return "walk" + std::to_string(flux_walk_len(dir, shape)) +
+               (dir == 0 && shape == FLUX_XWALK_SHAPE0 + 2 ? "d" : "");
     int W, H;
     flux_shape_dims(dir, shape, W, H);
     return std::to_string(W) + "x" + std::to_string(H);

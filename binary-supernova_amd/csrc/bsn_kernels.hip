@@ -373,6 +373,141 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
     }
 }
 
+// x interface fluxes as short register walks.  Tile (g, f0, j0, k | n): a
+// block of four waves stages rows j0 .. j0+3 of plane k, cells f0 .. f0+n+4
+// (padded x), in LDS; lane l of wave w walks row j0+w through its share of
+// the n <= 64 KMAX interfaces f0 .. f0+n-1 (KMAX consecutive ones when the
+// tile is full) with a six-cell register window read from LDS, and the split
+// form carries the smoothness indicators that consecutive interfaces share.
+// With OUT the fluxes go back through the row's LDS slots, so that the lanes
+// store them to HBM contiguously; without it each lane stores its fluxes
+// as they are formed (cached stores: the L2 joins the lanes' strided words
+// into whole lines), which needs no flux registers.  Cell b of a row sits at b + b/4: a lane's
+// cells start five words after its neighbour's, so the lanes of a wave read
+// distinct banks.  Interfaces 0 .. nx-1 are covered by such tiles; an edge
+// tile (g, -1, r0, 0) computes the last interface f = nx of the 256 rows
+// r0 .. r0+255 (row r = k ny + j), one per thread, gathering from HBM.
+template <typename T, int KMAX, bool OUT>
+__global__ __launch_bounds__(256) void k_fxwalk(const GridDev* __restrict__ grids,
+                                                const Tile* __restrict__ tiles, int ntiles) {
+    constexpr bool FAST = sizeof(T) == 4;        // split form
+    constexpr int NC = 64 * KMAX + 5;            // staged cells per row
+    constexpr int SB = NC + NC / 4 + 1;          // LDS row length after the skew
+    __shared__ T lds[5][4][SB];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    auto at = [](int b) { return b + (b >> 2); };
+    int t0, t1;
+    chunk(ntiles, t0, t1);
+    for (int t = t0; t < t1; ++t) {
+        const Tile tl = tiles[t];
+        const GridDev& G = grids[tl.g];
+        if (tl.a < 0) {
+            // edge tile: uniform over the block, so no thread reaches the
+            // barriers below while another takes this branch
+            const int r = tl.b + tid;
+            if (r < G.ny * G.nz) {
+                const int jr = r % G.ny, kr = r / G.ny;
+                T c[4][6], p[6];
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    double u[5];
+                    load_cell<0, true, FAST>(G, G.nx + m, NG + jr, NG + kr, u);
+                    const double pr = Prec<T>::pressure(u, G.gamma);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) c[q][m] = (T)u[q];
+                    p[m] = (T)pr;
+                }
+                T F[5];
+                Prec<T>::template flux<0>(c, p, G, F);
+                store_face<0>(G, F, G.nx, jr, kr);
+            }
+            continue;
+        }
+        const int f0 = tl.a, j0 = tl.b, k = tl.c & ZT_K0_MASK, n = tl.c >> ZT_LEN_SHIFT;
+        const int nc = n + 5;
+        for (int e = tid; e < 4 * nc; e += 256) {
+            const int a = e / nc, b = e - a * nc;
+            if (j0 + a < G.ny) {
+                double u[5];
+                load_cell<0, true, FAST>(G, f0 + b, NG + j0 + a, NG + k, u);
+                const double pr = Prec<T>::pressure(u, G.gamma);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) lds[q][a][at(b)] = (T)u[q];
+                lds[4][a][at(b)] = (T)pr;
+            }
+        }
+        __syncthreads();
+        const int j = j0 + w;
+        const int b0 = (lane * n) >> 6, len = (((lane + 1) * n) >> 6) - b0;
+        T Fb[OUT ? KMAX : 1][5];
+        if (j < G.ny && len > 0) {
+            T c[4][6], p[6];
+            auto place = [&](int slot, int b) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) c[q][slot] = lds[q][w][at(b)];
+                p[slot] = lds[4][w][at(b)];
+            };
+#pragma unroll
+            for (int m = 0; m < 5; ++m) place(m, b0 + m);
+            sp::WenoCarry wc[5];
+            if constexpr (FAST) sp::interface_walk_start(c, p, wc);
+            auto step = [&](int m, T (&F)[5]) {
+                place(5, b0 + m + 5);
+                if constexpr (FAST)
+                    sp::interface_flux<0, true>(c, p, G.gf32[0], G.gf32[1], G.gf32[2], F, wc);
+                else
+                    Prec<T>::template flux<0>(c, p, G, F);
+#pragma unroll
+                for (int s = 0; s < 5; ++s) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) c[q][s] = c[q][s + 1];
+                    p[s] = p[s + 1];
+                }
+            };
+            if constexpr (OUT) {
+                // unrolled, so that the flux registers are indexed statically
+#pragma unroll
+                for (int m = 0; m < KMAX; ++m)
+                    if (m < len) step(m, Fb[m]);
+            } else {
+                auto Fp = gmem((T*)G.F[0]);
+                const int64_t o = ((int64_t)k * G.ny + j) * G.fp[0] + f0 + b0;
+                for (int m = 0; m < len; ++m) {
+                    step(m, Fb[0]);
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) Fp[q * G.ffs[0] + o + m] = Fb[0][q];
+                }
+            }
+        }
+        if (!OUT) {
+            __syncthreads();
+            continue;
+        }
+        // every lane of the block has read its cells: the fluxes replace them
+        __syncthreads();
+        if (j < G.ny) {
+#pragma unroll
+            for (int m = 0; m < KMAX; ++m)
+                if (m < len) {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) lds[q][w][at(b0 + m)] = Fb[OUT ? m : 0][q];
+                }
+        }
+        __syncthreads();
+        if (j < G.ny) {
+            auto Fp = gmem((T*)G.F[0]);
+            const int64_t row = ((int64_t)k * G.ny + j) * G.fp[0] + f0;
+            // streaming stores, as for the other sweeps
+            for (int e = lane; e < n; e += 64) {
+#pragma unroll
+                for (int q = 0; q < 5; ++q)
+                    __builtin_nontemporal_store(lds[q][w][at(e)], &Fp[q * G.ffs[0] + row + e]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // y (DIR 1) and z (DIR 2) interface fluxes as register walks.  Tile
 // (g, i0, c0, f0 | len): 64 x 4 threads over (i, c) with c = k for DIR 1 and
 // c = j for DIR 2; each thread walks its column along DIR through the
@@ -1411,11 +1546,28 @@ static hipError_t launch_fwalk_t(const LaunchCtx& L, const GridDev* grids, const
     return hipGetLastError();
 }
 
+template <typename T, int KMAX, bool OUT>
+static hipError_t launch_fxwalk_t(const LaunchCtx& L, const GridDev* grids, const Tile* tiles,
+                                  int ntiles, int max_wg) {
+    if (ntiles <= 0) return hipSuccess;
+    const void* fn = (const void*)&k_fxwalk<T, KMAX, OUT>;
+    int g = occupancy_grid(fn, 256, 0, ntiles, L.cus);
+    if (max_wg > 0 && g > max_wg) g = max_wg;
+    hipLaunchKernelGGL((k_fxwalk<T, KMAX, OUT>), dim3(g), dim3(256), 0, L.stream, grids, tiles, ntiles);
+    return hipGetLastError();
+}
+
 template <int DIR, typename T>
 static hipError_t launch_flux_shape(const LaunchCtx& L, int shape, const GridDev* grids,
                                     const Tile* tiles, int ntiles, int max_wg) {
     if (DIR != 0 && shape >= FLUX_WALK_SHAPE0)
         return launch_fwalk_t<DIR == 0 ? 1 : DIR, T>(L, grids, tiles, ntiles, max_wg);
+    if (DIR == 0 && shape >= FLUX_XWALK_SHAPE0)
+        switch (shape - FLUX_XWALK_SHAPE0) {
+            case 0: return launch_fxwalk_t<T, 2, true>(L, grids, tiles, ntiles, max_wg);
+            case 1: return launch_fxwalk_t<T, 4, true>(L, grids, tiles, ntiles, max_wg);
+            default: return launch_fxwalk_t<T, 4, false>(L, grids, tiles, ntiles, max_wg);
+        }
     // shape ids: x sweep 0: 64x4, 1: 32x8, 2: 16x16;  y/z sweep 0: 64x4, 1: 64x8, 2: 32x8, 3: 64x2,
     // 4: 32x16
     if (DIR == 0) {
@@ -1435,23 +1587,29 @@ static hipError_t launch_flux_shape(const LaunchCtx& L, int shape, const GridDev
     }
 }
 
-// W x H of a tiled shape; for the walk shapes of the y and z sweeps W = 64
-// columns along x, H = 4 columns across, and the run length is
-// flux_walk_len(shape).
+// W x H of a tiled shape; for the walk shapes W = 64 lanes and H = 4 rows or
+// columns, with the run length flux_walk_len(dir, shape).
 void flux_shape_dims(int dir, int shape, int& W, int& H) {
     static const int xs[3][2] = {{64, 4}, {32, 8}, {16, 16}};
     static const int ys[5][2] = {{64, 4}, {64, 8}, {32, 8}, {64, 2}, {32, 16}};
-    if (dir != 0 && shape >= FLUX_WALK_SHAPE0 && shape < flux_shape_count(dir)) { W = 64; H = 4; return; }
+    if (flux_walk_len(dir, shape) > 0) { W = 64; H = 4; return; }
     if (dir == 0) { if (shape < 0 || shape > 2) shape = 0; W = xs[shape][0]; H = xs[shape][1]; }
     else          { if (shape < 0 || shape > 4) shape = 0; W = ys[shape][0]; H = ys[shape][1]; }
 }
 
-int flux_walk_len(int shape) {
+// Run length of a walk shape (interfaces per thread; for x the most one lane
+// takes), 0 for a tiled shape.
+int flux_walk_len(int dir, int shape) {
+    if (shape < 0 || shape >= flux_shape_count(dir)) return 0;
+    if (dir == 0) {
+        static const int lens[3] = {2, 4, 4};
+        return shape >= FLUX_XWALK_SHAPE0 ? lens[shape - FLUX_XWALK_SHAPE0] : 0;
+    }
     static const int lens[3] = {16, 32, 64};
-    return lens[shape - FLUX_WALK_SHAPE0];
+    return shape >= FLUX_WALK_SHAPE0 ? lens[shape - FLUX_WALK_SHAPE0] : 0;
 }
 
-int flux_shape_count(int dir) { return dir == 0 ? 3 : FLUX_WALK_SHAPE0 + 3; }
+int flux_shape_count(int dir) { return dir == 0 ? FLUX_XWALK_SHAPE0 + 3 : FLUX_WALK_SHAPE0 + 3; }
 
 hipError_t launch_flux(const LaunchCtx& L, int dir, int prec, int shape, const GridDev* grids,
                        const Tile* tiles, int ntiles, int max_wg) {

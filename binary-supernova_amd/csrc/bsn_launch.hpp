@@ -12,12 +12,15 @@ struct LaunchCtx {
     int cus;          // compute units of the device
 };
 
-// Flux sweep shapes: x 0..2 (tiled); y and z 0..4 tiled, then
-// FLUX_WALK_SHAPE0 .. +2 register walks of 16, 32 or 64 interfaces.
+// Flux sweep shapes: x 0..2 tiled, then FLUX_XWALK_SHAPE0 .. +2 short walks
+// of 2 or 4 interfaces per lane (fluxes stored through LDS) and of 4 (stored
+// directly); y and z 0..4 tiled, then FLUX_WALK_SHAPE0 .. +2 register walks
+// of 16, 32 or 64 interfaces.
+#define FLUX_XWALK_SHAPE0 3
 #define FLUX_WALK_SHAPE0 5
 void flux_shape_dims(int dir, int shape, int& W, int& H);
 int flux_shape_count(int dir);
-int flux_walk_len(int shape);
+int flux_walk_len(int dir, int shape);
 
 hipError_t launch_flux(const LaunchCtx& L, int dir, int prec, int shape, const GridDev* grids,
                        const Tile* tiles, int ntiles, int max_wg);

@@ -225,7 +225,9 @@ int bsn_batch_destroy(bsn_batch* b);
  * the shapes the autotuner keeps for a 256^3 box: split x 0, y/z 2, fused
  * kernel 11; FP64 x 2, y/z 2, fused kernel 2. */
 int bsn_batch_autotune(bsn_batch* b, char* report, int report_len);
-/* Sets the flux shape ids explicitly (x: 0..2 tiled; y/z: 0..4 tiled, 5..7
+/* Sets the flux shape ids explicitly (x: 0..2 tiled, 3..5 walks of 2, 4 and 4
+ * interfaces per lane through LDS-staged rows, the last storing its fluxes
+ * directly; y/z: 0..4 tiled, 5..7
  * register walks along the sweep of 16, 32 or 64 interfaces per thread). */
 int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz);
 /* Sets the shape id (0..11) of the fused z-sweep + stage-update kernel used

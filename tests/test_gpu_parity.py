@@ -152,9 +152,12 @@ def test_floors_and_ghosts_bitwise():
 
 
 @pytest.mark.parametrize("direction", ["x", "y", "z"])
-def test_sod_run_bitwise(direction):
-    print(f"sod {direction}", flush=True)
-    U, (nx, ny, nz), (dx, dy, dz) = P.sod_3d(direction, 128, 4, 1.4)
+@pytest.mark.parametrize("ntrans", [4, 8])
+def test_sod_run_bitwise(direction, ntrans):
+    # 128 x 4 x 4 is the reference's test (test/test_euler3d.jl), 128 x 8 x 8
+    # the Sod workload's size
+    print(f"sod {direction} {ntrans}", flush=True)
+    U, (nx, ny, nz), (dx, dy, dz) = P.sod_3d(direction, 128, ntrans, 1.4)
     g = bsn.Grid(nx, ny, nz).set_state(U)
     t = 0.0
     while t < 0.2:
@@ -491,10 +494,11 @@ def test_every_tile_shape_gives_the_same_bits(prec):
     s = (40, 36, 20)
     U = random_state(*s, seed=21)
     ref = None
-    # x shapes 0..2, y/z shapes 0..4 tiled and 5..7 register walks of 16, 32
-    # and 64 interfaces (runs end ragged on 37 and 21 interfaces); the z shape
-    # acts in the in-place form
-    for sx in range(3):
+    # x shapes 0..2 tiled and 3..5 short walks (40 interfaces per row in one
+    # partial tile, lanes take 0 or 1, and the edge tiles), y/z shapes 0..4
+    # tiled and 5..7 register walks of 16, 32 and 64 interfaces (runs end
+    # ragged on 37 and 21 interfaces); the z shape acts in the in-place form
+    for sx in range(6):
         for sy in range(8):
             for inplace in (False, True):
                 g = bsn.Grid(*s, prec).set_state(U)
