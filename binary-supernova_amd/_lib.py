@@ -33,6 +33,8 @@ EXPORTS = [
     "bsn_batch_run", "bsn_batch_times", "bsn_fmr_create", "bsn_fmr_destroy",
     "bsn_fmr_prolong_all", "bsn_fmr_prolong_fine_ghosts", "bsn_fmr_restrict", "bsn_fmr3d_step",
     "bsn_cfl_dt_fmr3d", "bsn_fmr_run", "bsn_fmr_time", "bsn_fmr_set_mode", "bsn_fmr_autotune", "bsn_batch_profile",
+    "bsn_fmr3d_step_gravity", "bsn_fmr_set_self_gravity", "bsn_solve_fmr_poisson",
+    "bsn_fmr_restrict_density", "bsn_fmr_prolong_scalar",
 ]
 
 
@@ -118,6 +120,11 @@ def load():
         "bsn_fmr_run": ([_p, _d, _d, _i, _i], _i),
         "bsn_fmr_time": ([_p, _pd, _pd], _i),
         "bsn_fmr_set_mode": ([_p, _i], _i),
+        "bsn_fmr3d_step_gravity": ([_p, _d, _i, _d], _i),
+        "bsn_fmr_set_self_gravity": ([_p, _i, _d], _i),
+        "bsn_solve_fmr_poisson": ([_p, C.POINTER(_p), _i], _i),
+        "bsn_fmr_restrict_density": ([_p, _i], _i),
+        "bsn_fmr_prolong_scalar": ([_p, _i, _p, _i, _p, _i], _i),
         "bsn_fmr_autotune": ([_p, C.c_char_p, _i], _i),
     }
     for name, (args, res) in sig.items():

@@ -137,6 +137,11 @@ struct bsn_fmr {
     hipGraph_t graph = nullptr;
     int inplace = 0;                 // 1: levels step in place with all fluxes materialised
     std::vector<GridDev*> d_fst;     // per level, the three stage tables of the fused step
+    // self-gravity (fmr3d_step! with self_gravity = true, src/fmr3d.jl:451-473):
+    // the frozen per-level phi of the root step, in buffers of its own
+    int sg_on = 0;                   // bsn_fmr_run solves it every root step
+    double sg_G = 1.0;
+    std::vector<double*> d_sgphi;
 };
 
 // ---------------------------------------------------------------------------
@@ -1736,6 +1741,7 @@ extern "C" int bsn_fmr_destroy(bsn_fmr* m) {
         (void)hipFree(f.frc); (void)hipFree(f.frf); (void)hipFree(f.Uci); (void)hipFree(f.Ucit);
     }
     for (auto* t : m->d_fst) (void)hipFree(t);
+    for (auto* p : m->d_sgphi) (void)hipFree(p);
     for (auto* g : m->lev) g->h.nzkeep = 0;
     (void)hipFree(m->d_f);
     (void)hipFree(m->d_all);
@@ -1934,6 +1940,140 @@ extern "C" int bsn_fmr3d_step(bsn_fmr* m, double dt_c) {
     return sync_ctx(m->ctx);
 }
 
+// ---------------------------------------------------------------------------
+// FMR self-gravity (src/fmr_gravity.jl:40-133, src/fmr3d.jl:451-473)
+
+static int64_t box_cells(const FmrDev& f) {
+    return (int64_t)(f.hi[0] - f.lo[0] + 1) * (f.hi[1] - f.lo[1] + 1) * (f.hi[2] - f.lo[2] + 1);
+}
+
+static int64_t active_cells(const GridDev& h) { return (int64_t)h.nx * h.ny * h.nz; }
+
+// Per-level phi buffers of the self-gravity solve.
+static int fmr_sg_buffers(bsn_fmr* m) {
+    if ((int)m->d_sgphi.size() != m->nlev) m->d_sgphi.assign(m->nlev, nullptr);
+    for (int l = 0; l < m->nlev; ++l)
+        if (!m->d_sgphi[l]) HIPCHK(hipMalloc((void**)&m->d_sgphi[l], active_cells(m->lev[l]->h) * 8));
+    bsn_grid* root = m->lev[0];
+    CHK(ensure_poisson(root, root->dx, root->dy, root->dz));
+    return BSN_OK;
+}
+
+// solve_fmr_poisson over the chain of levels into phi[l]: the density of
+// every level is restricted into its parent from the finest up (which
+// changes the parent density inside the child's box, as
+// restrict_density_to_coarse! does), the root's active density is solved
+// with isolated boundaries, and phi is prolonged trilinearly down the chain;
+// with G_grav != 1 every level's phi is then scaled.  With two levels this is
+// the reference's function.  The device tables must be current.
+static int fmr_sg_solve(bsn_fmr* m, double* const* phi, double G_grav) {
+    const LaunchCtx L = m->ctx->L();
+    for (int l = m->nlev - 1; l >= 1; --l)
+        HIPCHK(launch_restrict(L, m->d_all, m->d_f + (l - 1), box_cells(m->hf[l - 1]), 1));
+    bsn_grid* root = m->lev[0];
+    HIPCHK(poisson_solve_state(*root->pois, root->h, 0, phi[0], L.stream));
+    for (int l = 1; l < m->nlev; ++l) {
+        const GridDev& pc = m->lev[l - 1]->h;
+        const GridDev& cf = m->lev[l]->h;
+        const FmrDev& f = m->hf[l - 1];
+        const int nc[3] = {pc.nx, pc.ny, pc.nz}, nf[3] = {cf.nx, cf.ny, cf.nz};
+        HIPCHK(launch_prolong_scalar(L, phi[l - 1], nc, phi[l], nf, f.lo, f.ratio));
+    }
+    if (G_grav != 1.0)
+        for (int l = 0; l < m->nlev; ++l) HIPCHK(launch_scale(L, phi[l], active_cells(m->lev[l]->h), G_grav));
+    return BSN_OK;
+}
+
+// The levels read the frozen phi of the solve (restored to their own phi,
+// or none, by fmr_sg_end; the device tables follow at the next refresh).
+static int fmr_sg_begin(bsn_fmr* m, std::vector<const double*>& saved) {
+    CHK(fmr_sg_buffers(m));
+    saved.clear();
+    for (int l = 0; l < m->nlev; ++l) {
+        saved.push_back(m->lev[l]->h.phi);
+        m->lev[l]->h.phi = m->d_sgphi[l];
+    }
+    return fmr_refresh(m);
+}
+
+static void fmr_sg_end(bsn_fmr* m, const std::vector<const double*>& saved) {
+    for (int l = 0; l < m->nlev && l < (int)saved.size(); ++l) m->lev[l]->h.phi = saved[l];
+}
+
+extern "C" int bsn_fmr3d_step_gravity(bsn_fmr* m, double dt_c, int self_gravity, double G_grav) {
+    if (!m || (self_gravity != 0 && self_gravity != 1)) return set_err(BSN_ERR_ARG, "bad arguments");
+    if (!self_gravity) return bsn_fmr3d_step(m, dt_c);
+    if (!(G_grav == G_grav)) return set_err(BSN_ERR_ARG, "G_grav is NaN");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    std::vector<const double*> saved;
+    int r = fmr_sg_begin(m, saved);
+    if (r == BSN_OK) r = fmr_sg_solve(m, m->d_sgphi.data(), G_grav);
+    if (r == BSN_OK) {
+        hipError_t e = launch_set_scal(m->ctx->L(), m->lev[0]->h.scal, SCAL_DT, dt_c);
+        r = e == hipSuccess ? fmr_advance(m, 0) : set_err(BSN_ERR_HIP, "%s", hipGetErrorString(e));
+    }
+    if (r == BSN_OK) r = sync_ctx(m->ctx);
+    fmr_sg_end(m, saved);
+    return r;
+}
+
+extern "C" int bsn_solve_fmr_poisson(bsn_fmr* m, double* const* phi, int where) {
+    if (!m || !phi) return set_err(BSN_ERR_ARG, "NULL argument");
+    for (int l = 0; l < m->nlev; ++l)
+        if (!phi[l]) return set_err(BSN_ERR_ARG, "NULL phi for level %d", l);
+    HIPCHK(hipSetDevice(m->ctx->device));
+    CHK(fmr_refresh(m));
+    CHK(fmr_sg_buffers(m));
+    std::vector<double*> dst(m->nlev);
+    for (int l = 0; l < m->nlev; ++l) dst[l] = where == BSN_HOST ? m->d_sgphi[l] : phi[l];
+    CHK(fmr_sg_solve(m, dst.data(), 1.0));
+    if (where == BSN_HOST)
+        for (int l = 0; l < m->nlev; ++l)
+            HIPCHK(hipMemcpyAsync(phi[l], dst[l], active_cells(m->lev[l]->h) * 8, hipMemcpyDeviceToHost,
+                                  m->ctx->stream));
+    return sync_ctx(m->ctx);
+}
+
+extern "C" int bsn_fmr_restrict_density(bsn_fmr* m, int level) {
+    if (!m || level < 1 || level >= m->nlev) return set_err(BSN_ERR_ARG, "bad level");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    CHK(fmr_refresh(m));
+    HIPCHK(launch_restrict(m->ctx->L(), m->d_all, m->d_f + (level - 1), box_cells(m->hf[level - 1]), 1));
+    return sync_ctx(m->ctx);
+}
+
+extern "C" int bsn_fmr_prolong_scalar(bsn_fmr* m, int level, const double* phi_c, int where_c,
+                                      double* phi_f, int where_f) {
+    if (!m || level < 1 || level >= m->nlev) return set_err(BSN_ERR_ARG, "bad level");
+    if (!phi_c || !phi_f) return set_err(BSN_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    CHK(fmr_sg_buffers(m));
+    const GridDev& pc = m->lev[level - 1]->h;
+    const GridDev& cf = m->lev[level]->h;
+    const FmrDev& f = m->hf[level - 1];
+    const int64_t ncn = active_cells(pc), nfn = active_cells(cf);
+    // host arrays are staged through the self-gravity buffers of the two levels
+    const double* src = phi_c;
+    double* dst = where_f == BSN_HOST ? m->d_sgphi[level] : phi_f;
+    if (where_c == BSN_HOST) {
+        HIPCHK(hipMemcpyAsync(m->d_sgphi[level - 1], phi_c, ncn * 8, hipMemcpyHostToDevice, m->ctx->stream));
+        src = m->d_sgphi[level - 1];
+    }
+    const int nc[3] = {pc.nx, pc.ny, pc.nz}, nf[3] = {cf.nx, cf.ny, cf.nz};
+    HIPCHK(launch_prolong_scalar(m->ctx->L(), src, nc, dst, nf, f.lo, f.ratio));
+    if (where_f == BSN_HOST)
+        HIPCHK(hipMemcpyAsync(phi_f, dst, nfn * 8, hipMemcpyDeviceToHost, m->ctx->stream));
+    return sync_ctx(m->ctx);
+}
+
+extern "C" int bsn_fmr_set_self_gravity(bsn_fmr* m, int on, double G_grav) {
+    if (!m || (on != 0 && on != 1) || !(G_grav == G_grav)) return set_err(BSN_ERR_ARG, "bad arguments");
+    m->sg_on = on;
+    m->sg_G = G_grav;
+    graph_drop(m->gexec, m->graph);
+    return BSN_OK;
+}
+
 // Level CFL dts folded into the root: dt_0 = min(dt_0, r0 dt_1, r0 r1 dt_2, ...)
 static int fmr_cfl(bsn_fmr* m, double cfl, double t_end) {
     const LaunchCtx L = m->ctx->L();
@@ -1959,8 +2099,11 @@ extern "C" int bsn_cfl_dt_fmr3d(bsn_fmr* m, double cfl, double* dt_out) {
     return sync_ctx(m->ctx);
 }
 
+// One device-dt root step: the dt of the state at the start of the step
+// (cfl_dt_fmr3d), then with self-gravity the composite solve, then the step.
 static int fmr_device_step(bsn_fmr* m, double cfl, double t_end) {
     CHK(fmr_cfl(m, cfl, t_end));
+    if (m->sg_on) CHK(fmr_sg_solve(m, m->d_sgphi.data(), m->sg_G));
     CHK(fmr_advance(m, 0));
     HIPCHK(launch_advance_t(m->ctx->L(), m->d_all, 1));
     return BSN_OK;
@@ -1969,6 +2112,19 @@ static int fmr_device_step(bsn_fmr* m, double cfl, double t_end) {
 extern "C" int bsn_fmr_run(bsn_fmr* m, double cfl, double t_end, int nsteps, int use_graph) {
     if (!m || nsteps < 0) return set_err(BSN_ERR_ARG, "bad arguments");
     HIPCHK(hipSetDevice(m->ctx->device));
+    if (m->sg_on) {
+        // the Poisson solve's FFT plans are launched eagerly: no graph
+        std::vector<const double*> saved;
+        int r = fmr_sg_begin(m, saved);
+        if (r == BSN_OK) {
+            hipError_t e = launch_set_scal(m->ctx->L(), m->lev[0]->h.scal, SCAL_T, 0.0);
+            if (e != hipSuccess) r = set_err(BSN_ERR_HIP, "%s", hipGetErrorString(e));
+        }
+        for (int s = 0; s < nsteps && r == BSN_OK; ++s) r = fmr_device_step(m, cfl, t_end);
+        if (r == BSN_OK) r = sync_ctx(m->ctx);
+        fmr_sg_end(m, saved);
+        return r;
+    }
     CHK(fmr_refresh(m));
     const LaunchCtx L = m->ctx->L();
     HIPCHK(launch_set_scal(L, m->lev[0]->h.scal, SCAL_T, 0.0));

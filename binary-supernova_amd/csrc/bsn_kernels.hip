@@ -1348,7 +1348,9 @@ __global__ void k_time_interp(const GridDev* __restrict__ grids, const FmrDev* _
 }
 
 // _restrict! (fmr3d.jl:266-285)
-__global__ void k_restrict(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr) {
+// _restrict! (fields 0..nq-1; nq = 5) and restrict_density_to_coarse!
+// (nq = 1, src/fmr_gravity.jl:40-60): the same sum in the same order.
+__global__ void k_restrict(const GridDev* __restrict__ grids, const FmrDev* __restrict__ fmr, int nq) {
     const FmrDev& M = *fmr;
     const GridDev& C = grids[M.parent];
     const GridDev& Fg = grids[M.child];
@@ -1362,7 +1364,7 @@ __global__ void k_restrict(const GridDev* __restrict__ grids, const FmrDev* __re
         const int64_t rr = e / nic;
         const int n = (int)(rr % njc), p = (int)(rr / njc);
         const int64_t dc = uidx(C, NG + M.lo[0] + m - 1, NG + M.lo[1] + n - 1, NG + M.lo[2] + p - 1);
-        for (int q = 0; q < 5; ++q) {
+        for (int q = 0; q < nq; ++q) {
             double val = 0.0;
             for (int sx = 1; sx <= r; ++sx)
                 for (int sy = 1; sy <= r; ++sy)
@@ -1372,6 +1374,50 @@ __global__ void k_restrict(const GridDev* __restrict__ grids, const FmrDev* __re
             C.U[q * C.fs + dc] = val * inv_r3;
         }
     }
+}
+
+// _prolong_scalar_trilinear! (src/fmr_gravity.jl:70-106): phi of the fine
+// active cells (x fastest) from the coarse active-cell phi, the fine box
+// starting at coarse active cell (lo_x, lo_y, lo_z) (1-based), in the
+// reference's operation order.
+__global__ void k_prolong_scalar(const double* __restrict__ phc, int ncx, int ncy, int ncz,
+                                 double* __restrict__ phf, int nfx, int nfy, int nfz, int lox,
+                                 int loy, int loz, int r) {
+    const int64_t total = (int64_t)nfx * nfy * nfz;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % nfx);
+        const int64_t rr = e / nfx;
+        const int j = (int)(rr % nfy), k = (int)(rr / nfy);
+        // fine cell centre in coarse active-cell coordinates (1-based)
+        const double xc = ((double)(lox - 1) + ((double)(i + 1) - 0.5) / (double)r) + 0.5;
+        const double yc = ((double)(loy - 1) + ((double)(j + 1) - 0.5) / (double)r) + 0.5;
+        const double zc = ((double)(loz - 1) + ((double)(k + 1) - 0.5) / (double)r) + 0.5;
+        const int ic = min(max((int)floor(xc), 1), ncx - 1);
+        const int jc = min(max((int)floor(yc), 1), ncy - 1);
+        const int kc = min(max((int)floor(zc), 1), ncz - 1);
+        const double fx = xc - ic, fy = yc - jc, fz = zc - kc;
+        const double w000 = (1 - fx) * (1 - fy) * (1 - fz);
+        const double w100 = fx * (1 - fy) * (1 - fz);
+        const double w010 = (1 - fx) * fy * (1 - fz);
+        const double w110 = fx * fy * (1 - fz);
+        const double w001 = (1 - fx) * (1 - fy) * fz;
+        const double w101 = fx * (1 - fy) * fz;
+        const double w011 = (1 - fx) * fy * fz;
+        const double w111 = fx * fy * fz;
+        auto P = [&](int a, int b, int c) {
+            return phc[((int64_t)(kc - 1 + c) * ncy + (jc - 1 + b)) * ncx + (ic - 1 + a)];
+        };
+        phf[e] = w000 * P(0, 0, 0) + w100 * P(1, 0, 0) + w010 * P(0, 1, 0) + w110 * P(1, 1, 0) +
+                 w001 * P(0, 0, 1) + w101 * P(1, 0, 1) + w011 * P(0, 1, 1) + w111 * P(1, 1, 1);
+    }
+}
+
+// a .*= s
+__global__ void k_scale(double* __restrict__ a, int64_t n, double s) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        a[e] = a[e] * s;
 }
 
 // Flux registers: face f in 0..5 = x_lo, x_hi, y_lo, y_hi, z_lo, z_hi, each
@@ -1835,8 +1881,25 @@ hipError_t launch_time_interp(const LaunchCtx& L, const GridDev* grids, const Fm
     return hipGetLastError();
 }
 
-hipError_t launch_restrict(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total) {
-    hipLaunchKernelGGL(k_restrict, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids, fmr);
+hipError_t launch_restrict(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total,
+                           int nfields) {
+    hipLaunchKernelGGL(k_restrict, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, grids, fmr,
+                       nfields);
+    return hipGetLastError();
+}
+
+hipError_t launch_prolong_scalar(const LaunchCtx& L, const double* phc, const int nc[3], double* phf,
+                                 const int nf[3], const int lo[3], int ratio) {
+    const int64_t total = (int64_t)nf[0] * nf[1] * nf[2];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prolong_scalar, dim3(flat_blocks(total, L.cus)), dim3(256), 0, L.stream, phc,
+                       nc[0], nc[1], nc[2], phf, nf[0], nf[1], nf[2], lo[0], lo[1], lo[2], ratio);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(const LaunchCtx& L, double* a, int64_t n, double s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale, dim3(flat_blocks(n, L.cus)), dim3(256), 0, L.stream, a, n, s);
     return hipGetLastError();
 }
 

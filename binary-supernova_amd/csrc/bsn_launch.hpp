@@ -64,7 +64,14 @@ hipError_t launch_prolong(const LaunchCtx& L, const GridDev* grids, const FmrDev
                           ProlongDst dst = ProlongDst{0, {}});
 hipError_t launch_time_interp(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr,
                               int64_t total, double alpha);
-hipError_t launch_restrict(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total);
+// _restrict! (nfields = 5) or restrict_density_to_coarse! (nfields = 1)
+hipError_t launch_restrict(const LaunchCtx& L, const GridDev* grids, const FmrDev* fmr, int64_t total,
+                           int nfields = 5);
+// _prolong_scalar_trilinear! of active-cell phi (x fastest), box at
+// coarse active cell lo (1-based)
+hipError_t launch_prolong_scalar(const LaunchCtx& L, const double* phc, const int nc[3], double* phf,
+                                 const int nf[3], const int lo[3], int ratio);
+hipError_t launch_scale(const LaunchCtx& L, double* a, int64_t n, double s);
 hipError_t launch_fr_zero(const LaunchCtx& L, const FmrDev* fmr, int64_t total);
 hipError_t launch_fr_coarse(const LaunchCtx& L, int prec, const GridDev* grids, const FmrDev* fmr,
                             int64_t nelem, double weight);

@@ -10,7 +10,7 @@ import json
 
 import numpy as np
 
-from ._lib import call
+from ._lib import BSN_HOST, call
 from .euler3d import Context, Grid, bc_mask
 
 NG = 3
@@ -77,8 +77,39 @@ class FMRHierarchy:
         """_restrict! (src/fmr3d.jl:266-285)"""
         call("bsn_fmr_restrict", self.handle, level)
 
-    def step(self, dt_c: float) -> None:
-        call("bsn_fmr3d_step", self.handle, float(dt_c))
+    def step(self, dt_c: float, self_gravity: bool = False, G_grav: float = 1.0) -> None:
+        """fmr3d_step!(G, dt_c; self_gravity, G_grav): with self_gravity the
+        composite potential is solved once and frozen for the root step."""
+        call("bsn_fmr3d_step_gravity", self.handle, float(dt_c), int(bool(self_gravity)),
+             float(G_grav))
+
+    def set_self_gravity(self, on: bool = True, G_grav: float = 1.0) -> None:
+        """The frozen-phi self-gravity solve at every root step of run()."""
+        call("bsn_fmr_set_self_gravity", self.handle, int(bool(on)), float(G_grav))
+
+    def solve_poisson(self) -> list:
+        """solve_fmr_poisson: per-level active-cell phi, (nz, ny, nx) each;
+        restricts the density into the parents first."""
+        outs = [np.empty((nz, ny, nx)) for (nx, ny, nz) in self.dims]
+        arr = (C.c_void_p * len(outs))(*[o.ctypes.data for o in outs])
+        call("bsn_solve_fmr_poisson", self.handle, arr, BSN_HOST)
+        return outs
+
+    def restrict_density(self, level: int) -> None:
+        """restrict_density_to_coarse! of `level` into its parent"""
+        call("bsn_fmr_restrict_density", self.handle, level)
+
+    def prolong_scalar(self, level: int, phi_c: np.ndarray) -> np.ndarray:
+        """_prolong_scalar_trilinear! of level-1's phi (nz, ny, nx) onto `level`"""
+        pc = self.dims[level - 1]
+        phi_c = np.ascontiguousarray(phi_c, dtype=np.float64)
+        if phi_c.shape != (pc[2], pc[1], pc[0]):
+            raise ValueError(f"phi_c shape {phi_c.shape} != {(pc[2], pc[1], pc[0])}")
+        nx, ny, nz = self.dims[level]
+        out = np.empty((nz, ny, nx))
+        call("bsn_fmr_prolong_scalar", self.handle, level, C.c_void_p(phi_c.ctypes.data), BSN_HOST,
+             C.c_void_p(out.ctypes.data), BSN_HOST)
+        return out
 
     def cfl_dt(self, cfl: float) -> float:
         out = C.c_double()
@@ -124,10 +155,29 @@ class FMRGrid3D(FMRHierarchy):
         self.prolong_all(1)
 
 
-def fmr3d_step_(G: FMRHierarchy, dt_c: float) -> None:
-    """fmr3d_step! (src/fmr3d.jl:451-604); a level's phi, when set, is the
-    frozen potential of the self_gravity branch."""
-    G.step(dt_c)
+def fmr3d_step_(G: FMRHierarchy, dt_c: float, *, self_gravity: bool = False,
+                G_grav: float = 1.0) -> None:
+    """fmr3d_step!(G, dt_c; self_gravity=false, G_grav=1.0) (src/fmr3d.jl:
+    451-604).  Without self_gravity a level's phi, when set, is a static
+    potential; with it the composite potential of solve_fmr_poisson, times
+    G_grav, is frozen across the root step."""
+    G.step(dt_c, self_gravity, G_grav)
+
+
+def solve_fmr_poisson(G: FMRHierarchy) -> list:
+    """solve_fmr_poisson(G) (src/fmr_gravity.jl:119-133): per-level phi."""
+    return G.solve_poisson()
+
+
+def restrict_density_to_coarse_(G: FMRHierarchy, level: int = 1) -> None:
+    """restrict_density_to_coarse!(G) (src/fmr_gravity.jl:40-60)"""
+    G.restrict_density(level)
+
+
+def _prolong_scalar_trilinear_(phi_f: np.ndarray, phi_c: np.ndarray, G: FMRHierarchy,
+                               level: int = 1) -> None:
+    """_prolong_scalar_trilinear!(phi_f, phi_c, G) (src/fmr_gravity.jl:70-106)"""
+    phi_f[...] = G.prolong_scalar(level, phi_c)
 
 
 def cfl_dt_fmr3d(G: FMRHierarchy, cfl: float) -> float:

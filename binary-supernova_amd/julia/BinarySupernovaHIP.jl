@@ -18,6 +18,8 @@ module BinarySupernovaHIP
 export HIPGrid, euler3d_step!, euler3d_rhs!, cfl_dt_3d, apply_floors_3d!,
        fill_ghost_3d_outflow!, fill_ghost_3d_periodic!, rk3_step!, HIPFMR,
        fmr3d_step!, cfl_dt_fmr3d, fmr_run!, fmr_autotune!, fmr_set_mode!, set_phi!, download,
+       solve_fmr_poisson, restrict_density_to_coarse!, _prolong_scalar_trilinear!,
+       fmr_set_self_gravity!,
        add_bh_gravity_source!,
        add_sink_sources!, set_point_masses!, set_sinks!, solve_poisson_isolated,
        set_self_gravity!, gas_force_on_bh, accrete!, gas_diagnostics, gas_energy_total,
@@ -208,9 +210,42 @@ function set_params!(g::HIPGrid, dx, dy, dz, γ; bc::Symbol = :outflow, ρ_floor
                 g.handle, dx, dy, dz, γ, bcmask(bc), ρ_floor, P_floor))
 end
 
-"""fmr3d_step!(G, dt_c) -- each level's phi (set_phi!) is the frozen potential."""
-function fmr3d_step!(G::HIPFMR, dt_c::Float64)
-    check(ccall((:bsn_fmr3d_step, libbsn), Cint, (Ptr{Cvoid}, Float64), G.handle, dt_c))
+"""fmr3d_step!(G, dt_c; self_gravity = false, G_grav = 1.0) -- with self_gravity
+the composite potential (solve_fmr_poisson) times G_grav is frozen for the root
+step; without it a level's phi (set_phi!), when set, is a static potential."""
+function fmr3d_step!(G::HIPFMR, dt_c::Float64; self_gravity::Bool = false, G_grav::Real = 1.0)
+    check(ccall((:bsn_fmr3d_step_gravity, libbsn), Cint, (Ptr{Cvoid}, Float64, Cint, Float64),
+                G.handle, dt_c, self_gravity ? 1 : 0, G_grav))
+end
+
+"""solve_fmr_poisson(G) -- (Φ per level, root first), each (nx, ny, nz); the
+parents' density inside each child box is restricted first."""
+function solve_fmr_poisson(G::HIPFMR)
+    Φ = [zeros(g.nx, g.ny, g.nz) for g in G.levels]
+    check(ccall((:bsn_solve_fmr_poisson, libbsn), Cint, (Ptr{Cvoid}, Ptr{Ptr{Float64}}, Cint),
+                G.handle, [pointer(p) for p in Φ], BSN_HOST))
+    return Tuple(Φ)
+end
+
+"""restrict_density_to_coarse!(G; level = 1)"""
+function restrict_density_to_coarse!(G::HIPFMR; level::Int = 1)
+    check(ccall((:bsn_fmr_restrict_density, libbsn), Cint, (Ptr{Cvoid}, Cint), G.handle, level))
+end
+
+"""_prolong_scalar_trilinear!(Φ_f, Φ_c, G; level = 1)"""
+function _prolong_scalar_trilinear!(Φ_f::Array{Float64,3}, Φ_c::Array{Float64,3}, G::HIPFMR;
+                                    level::Int = 1)
+    check(ccall((:bsn_fmr_prolong_scalar, libbsn), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint),
+                G.handle, level, Φ_c, BSN_HOST, Φ_f, BSN_HOST))
+    return Φ_f
+end
+
+"""fmr_set_self_gravity!(G, on; G_grav = 1.0) -- the composite solve at every
+root step of fmr_run!."""
+function fmr_set_self_gravity!(G::HIPFMR, on::Bool; G_grav::Real = 1.0)
+    check(ccall((:bsn_fmr_set_self_gravity, libbsn), Cint, (Ptr{Cvoid}, Cint, Float64),
+                G.handle, on ? 1 : 0, G_grav))
 end
 
 function cfl_dt_fmr3d(G::HIPFMR, cfl::Real)

@@ -278,9 +278,34 @@ int bsn_fmr_restrict(bsn_fmr* m, int level);
 int bsn_fmr3d_step(bsn_fmr* m, double dt_c);
 /* cfl_dt_fmr3d(G, cfl) (src/fmr3d.jl:616). */
 int bsn_cfl_dt_fmr3d(bsn_fmr* m, double cfl, double* dt_out);
-/* nsteps CFL-limited root steps with dt on the device; use_graph as above. */
+/* nsteps CFL-limited root steps with dt on the device; use_graph as above
+ * (ignored while self-gravity is on: the Poisson solve is launched eagerly). */
 int bsn_fmr_run(bsn_fmr* m, double cfl, double t_end, int nsteps, int use_graph);
 int bsn_fmr_time(bsn_fmr* m, double* t, double* dt);
+
+/* FMR self-gravity (src/fmr_gravity.jl) ------------------------------------
+ * fmr3d_step!(G, dt_c; self_gravity, G_grav) (src/fmr3d.jl:451-473): with
+ * self_gravity = 1 the composite Poisson problem is solved once at the start
+ * of the root step (solve_fmr_poisson, which restricts the density into the
+ * parents first), every level's phi is scaled by G_grav and frozen across all
+ * stages and sub-cycles of the step; the levels' own phi (bsn_grid_set_phi)
+ * is not used during the step and is theirs again afterwards.  More than two
+ * levels: the density is restricted from the finest level up and phi is
+ * prolonged down the chain.  self_gravity = 0 is bsn_fmr3d_step. */
+int bsn_fmr3d_step_gravity(bsn_fmr* m, double dt_c, int self_gravity, double G_grav);
+/* The same frozen-phi solve at every root step of bsn_fmr_run (on = 0 stops). */
+int bsn_fmr_set_self_gravity(bsn_fmr* m, int on, double G_grav);
+/* solve_fmr_poisson(G) (src/fmr_gravity.jl:119-133): phi[l] receives level
+ * l's active-cell potential (nx*ny*nz, x fastest), host or device arrays
+ * (where); the parents' density inside each child box is restricted first. */
+int bsn_solve_fmr_poisson(bsn_fmr* m, double* const* phi, int where);
+/* restrict_density_to_coarse!(G) (src/fmr_gravity.jl:40-60) for level l >= 1
+ * into its parent (density only). */
+int bsn_fmr_restrict_density(bsn_fmr* m, int level);
+/* _prolong_scalar_trilinear!(phi_f, phi_c, G) (src/fmr_gravity.jl:70-106)
+ * from level l-1's active-cell array to level l's. */
+int bsn_fmr_prolong_scalar(bsn_fmr* m, int level, const double* phi_c, int where_c,
+                           double* phi_f, int where_f);
 
 /* Kernel timing of the last bsn_batch_run / bsn_fmr_run with profile=1:
  * average milliseconds per launch of the flux (x, y, z) and update kernels. */
