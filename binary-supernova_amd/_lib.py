@@ -34,7 +34,7 @@ EXPORTS = [
     "bsn_fmr_prolong_all", "bsn_fmr_prolong_fine_ghosts", "bsn_fmr_restrict", "bsn_fmr3d_step",
     "bsn_cfl_dt_fmr3d", "bsn_fmr_run", "bsn_fmr_time", "bsn_fmr_set_mode", "bsn_fmr_autotune", "bsn_batch_profile",
     "bsn_fmr3d_step_gravity", "bsn_fmr_set_self_gravity", "bsn_solve_fmr_poisson",
-    "bsn_fmr_restrict_density", "bsn_fmr_prolong_scalar",
+    "bsn_fmr_restrict_density", "bsn_fmr_prolong_scalar", "bsn_conserved_totals_noalloc",
 ]
 
 
@@ -85,6 +85,7 @@ def load():
         "bsn_rk3_save": ([_p], _i),
         "bsn_rk3_combine": ([_p, _i, _p, _i, _d], _i),
         "bsn_conserved_totals": ([_p, _d, _pd], _i),
+        "bsn_conserved_totals_noalloc": ([_p, _d, _p, _i], _i),
         "bsn_add_bh_gravity_source": ([_p, _p, _i, _d, _d, _d, _i, _pd, _pd, _pd, _d, _d, _d], _i),
         "bsn_add_sink_sources": ([_p, _p, _i, _d, _d, _d, _i, _pd, _pd, _pd, _pd, _i, _d, _d, _d, _d],
                                  _i),

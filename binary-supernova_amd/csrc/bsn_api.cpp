@@ -63,6 +63,7 @@ struct bsn_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int cus = 1;
+    double* totals_scratch = nullptr;   // device address of the fixed totals scratch
     char name[256] = {0};
     LaunchCtx L() const { return LaunchCtx{stream, cus}; }
 };
@@ -526,7 +527,12 @@ extern "C" int bsn_ctx_create(int device, bsn_ctx** out) {
     c->cus = prop.multiProcessorCount;
     snprintf(c->name, sizeof c->name, "%s (%s), %d CUs, %zu MB", prop.name, prop.gcnArchName,
              prop.multiProcessorCount, (size_t)(prop.totalGlobalMem >> 20));
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    hipError_t e = totals_scratch_setup(device, &c->totals_scratch);
+    if (e != hipSuccess) {
+        delete c;
+        return set_err(BSN_ERR_HIP, "address of the totals scratch: %s", hipGetErrorString(e));
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return set_err(BSN_ERR_HIP, "stream: %s", hipGetErrorString(e)); }
     *out = c;
     return BSN_OK;
@@ -1078,10 +1084,29 @@ extern "C" int bsn_rk3_combine(bsn_grid* g, int stage, const double* dU, int whe
 extern "C" int bsn_conserved_totals(bsn_grid* g, double dV, double out[5]) {
     if (!g || !out) return set_err(BSN_ERR_ARG, "NULL argument");
     HIPCHK(hipSetDevice(g->ctx->device));
-    const int nb = 1024;
+    const int nb = TOTALS_BLOCKS;
     if (!g->d_part) HIPCHK(hipMalloc((void**)&g->d_part, nb * 5 * 8));
     HIPCHK(launch_totals(g->ctx->L(), g->h, g->d_part, nb, dV, &g->h.scal[SCAL_TOT]));
     HIPCHK(hipMemcpyAsync(out, &g->h.scal[SCAL_TOT], 5 * 8, hipMemcpyDeviceToHost, g->ctx->stream));
+    return sync_ctx(g->ctx);
+}
+
+// The same sums with the partials in the fixed scratch of the grid's device
+// (set up by bsn_ctx_create): no device memory is allocated.  Every partial
+// slot the launch uses is written by its block before the final sum reads
+// it, so nothing needs clearing between calls.
+extern "C" int bsn_conserved_totals_noalloc(bsn_grid* g, double dV, double* out, int where) {
+    if (!g || !out) return set_err(BSN_ERR_ARG, "NULL argument");
+    if (where != BSN_HOST && where != BSN_DEVICE) return set_err(BSN_ERR_ARG, "where %d", where);
+    double* scr = g->ctx->totals_scratch;
+    if (!scr) return set_err(BSN_ERR_STATE, "no totals scratch on device %d", g->ctx->device);
+    // the launch uses TOTALS_BLOCKS x 5 partials and 5 sums: within the scratch
+    static_assert(5 * TOTALS_BLOCKS + 5 <= TOTALS_SCRATCH_DOUBLES, "totals scratch too small");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    double* sums = where == BSN_DEVICE ? out : scr + 5 * TOTALS_BLOCKS;
+    HIPCHK(launch_totals(g->ctx->L(), g->h, scr, TOTALS_BLOCKS, dV, sums));
+    if (where == BSN_DEVICE) return BSN_OK;
+    HIPCHK(hipMemcpyAsync(out, sums, 5 * 8, hipMemcpyDeviceToHost, g->ctx->stream));
     return sync_ctx(g->ctx);
 }
 

@@ -22,6 +22,7 @@
 // chunk of the list, which keeps neighbouring tiles (shared halo cells) on one
 // CU and one XCD L2.
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include "bsn_internal.hpp"
 #include "bsn_physics.hpp"
 #include "bsn_launch.hpp"
@@ -1856,6 +1857,31 @@ hipError_t launch_combine_all(const LaunchCtx& L, const GridDev& G, int stage, c
 hipError_t launch_copy_state(const LaunchCtx& L, const GridDev& G, const double* src, double* dst) {
     hipLaunchKernelGGL(k_copy_state, dim3(flat_blocks(5 * G.fs, L.cus)), dim3(256), 0, L.stream, G, src, dst);
     return hipGetLastError();
+}
+
+// Fixed scratch of bsn_conserved_totals_noalloc: the block partial sums of
+// one totals launch (TOTALS_BLOCKS x 5) and its five sums, in a file-scope
+// device array.  Each device holds its own copy until the process ends; host
+// code reaches it only through the address totals_scratch_setup returns
+// (the array's name is a host-side stand-in, not a device address).
+static __device__ __align__(256) double scratch[TOTALS_SCRATCH_DOUBLES];
+
+static std::mutex s_scratch_mu;
+static double* s_scratch_ptr[BSN_MAX_DEVICES] = {};
+
+// Once per device, with that device current: the device address of scratch.
+hipError_t totals_scratch_setup(int device, double** ptr) {
+    *ptr = nullptr;
+    if (device < 0 || device >= BSN_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(s_scratch_mu);
+    if (!s_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        s_scratch_ptr[device] = (double*)p;
+    }
+    *ptr = s_scratch_ptr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_totals(const LaunchCtx& L, const GridDev& G, double* part, int nblocks, double dV,

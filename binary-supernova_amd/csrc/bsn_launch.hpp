@@ -50,6 +50,13 @@ hipError_t launch_src(const LaunchCtx& L, const GridDev& G, double* dU, int whic
 hipError_t launch_combine_all(const LaunchCtx& L, const GridDev& G, int stage, const double* dU,
                               double dt);
 hipError_t launch_copy_state(const LaunchCtx& L, const GridDev& G, const double* src, double* dst);
+// Blocks of one conserved-totals launch (bsn_conserved_totals and its
+// allocation-free variant use the same count, hence the same sums), and the
+// size of the variant's fixed scratch: the partials, then the five sums.
+#define TOTALS_BLOCKS 1024
+#define TOTALS_SCRATCH_DOUBLES (5 * TOTALS_BLOCKS + 8)
+#define BSN_MAX_DEVICES 64
+hipError_t totals_scratch_setup(int device, double** ptr);
 hipError_t launch_totals(const LaunchCtx& L, const GridDev& G, double* part, int nblocks, double dV,
                          double* out);
 // Buffers (child layout) a prolongation writes; none = the child's state.

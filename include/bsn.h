@@ -107,6 +107,17 @@ int bsn_rk3_save(bsn_grid* g);
 int bsn_rk3_combine(bsn_grid* g, int stage, const double* dU, int where, double dt);
 /* Conservation totals sum(U[q, active]) * dV, q = 1..5 (src/diagnostics.jl:21-58). */
 int bsn_conserved_totals(bsn_grid* g, double dV, double out[5]);
+/* No counterpart in the reference: bsn_conserved_totals for callers that
+ * cannot allocate device memory (code under hipGraph capture, serving loops
+ * that forbid hipMalloc).  The 1024 block partial sums live in a fixed
+ * library array of 5 x 1024 + 8 doubles (41 KB) on each device, held until
+ * the process ends, instead of a buffer allocated on the first call; the sums
+ * are those of bsn_conserved_totals, bit for bit.  The scratch does not grow
+ * with the grid, so no grid is too large for it.  out: 5 doubles in host
+ * memory (where = BSN_HOST; the call waits for the stream) or device memory
+ * (BSN_DEVICE; nothing waits, so the call can be captured).  Calls that
+ * overlap on one device race on the array. */
+int bsn_conserved_totals_noalloc(bsn_grid* g, double dV, double* out, int where);
 
 /* Point masses and gas sinks ---------------------------------------------- */
 /* The bodies of the reference's BlackHole list (src/BinarySupernova.jl:130-137)
