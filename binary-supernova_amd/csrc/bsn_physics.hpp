@@ -498,9 +498,13 @@ __device__ __forceinline__ void interface_flux(const float (&c)[4][6], const flo
                                                float g, float gfac, float ig1, float F[5],
                                                WenoCarry (&w)[5]) {
     float L[4], R[4];
+    // one quantity at a time: interleaving all five reconstructions raises
+    // the register pressure past what the fused update kernel can hold
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 4; ++q) {
         weno5_pair<WALK>(c[q][0], c[q][1], c[q][2], c[q][3], c[q][4], c[q][5], w[q], L[q], R[q]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     float P_maxL = fmaxj(fmaxj(fmaxj(fmaxj(p[0], p[1]), p[2]), p[3]), p[4]);
     float P_maxR = fmaxj(fmaxj(fmaxj(fmaxj(p[1], p[2]), p[3]), p[4]), p[5]);
     float PLw, PRw;
