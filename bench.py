@@ -34,6 +34,9 @@ import numpy as np
 
 ROOT = pathlib.Path(__file__).resolve().parent
 LEAN = False
+# precision -> [x, y, z, fused] shape ids to use instead of the autotuner
+# (--shapes), so that a profiled run holds only the timed launches
+SHAPES: dict = {}
 
 # MI355X published peaks
 HBM_PEAK_GBS = 8000.0
@@ -98,7 +101,14 @@ def run_uniform(bsn, U, dims, d, gamma, bc, floors, prec, steps, warmup, cfl=0.4
     g = bsn.Grid(nx, ny, nz, prec).set_state(U)
     g.set_params(*d, gamma, bc, *floors)
     b = bsn.Batch([g])
-    tune = b.autotune()
+    if prec in SHAPES:
+        sx, sy, sz, su = SHAPES[prec]
+        b.set_shapes(sx, sy, sz)
+        b.set_update_shape(su)
+        tune = {"fixed": SHAPES[prec]}
+    else:
+        tune = b.autotune()
+    shape_ids = b.get_shapes()
     if warmup:
         b.run(cfl, warmup, graph=True)
         g.set_state(U)
@@ -118,7 +128,7 @@ def run_uniform(bsn, U, dims, d, gamma, bc, floors, prec, steps, warmup, cfl=0.4
         wall_graph = time.perf_counter() - t0
     zones = nx * ny * nz * steps
     return dict(wall=wall, wall_graph=wall_graph, zones=zones, ms_flux=ms_flux, ms_update=ms_upd,
-                ms_step=ms_step, tune=tune, state=out, grid=g, batch=b)
+                ms_step=ms_step, tune=tune, shape_ids=shape_ids, state=out, grid=g, batch=b)
 
 
 def totals(U, dV):
@@ -252,7 +262,7 @@ def bench_turb(bsn, args):
                     grid=[n, n, n], precision="split"),
         fp64=dict(value=v64, ms_per_step=1e3 * r64["wall"] / args.steps,
                   value_graph=r64["zones"] / r64["wall_graph"], roofline=upd64,
-                  roofline_flux=flux64, tiles=r64["tune"]),
+                  roofline_flux=flux64, tiles=r64["tune"], shape_ids=r64["shape_ids"]),
         split_over_fp64=v32 / v64,
         value_graph=r32["zones"] / r32["wall_graph"],
         l1_density_split_vs_fp64=L1,
@@ -260,7 +270,7 @@ def bench_turb(bsn, args):
         energy_drift_split=float(abs(tB[4] - t0[4]) / t0[4]),
         mass_drift_fp64=float(abs(tA[0] - t0[0]) / t0[0]),
         energy_drift_fp64=float(abs(tA[4] - t0[4]) / t0[4]),
-        roofline=upd, roofline_flux=flux, tiles=r32["tune"],
+        roofline=upd, roofline_flux=flux, tiles=r32["tune"], shape_ids=r32["shape_ids"],
         kernel_ms=dict(flux_xyz=r32["ms_flux"], update=r32["ms_update"],
                        fp64_flux_xyz=r64["ms_flux"], fp64_update=r64["ms_update"]),
         copy_calibration_bytes=copy_bytes,
@@ -269,7 +279,8 @@ def bench_turb(bsn, args):
     log("stand-alone update kernel (in-place form)")
     out["roofline_update"] = standalone_update_roofline(r32, "split", cells)
     out["fp64"]["roofline_update"] = standalone_update_roofline(r64, "fp64", cells)
-    out["pcie_inclusive"] = host_array_rate(bsn, U, dims, d, gamma, r32["tune"])
+    if not LEAN:
+        out["pcie_inclusive"] = host_array_rate(bsn, U, dims, d, gamma, r32["tune"])
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(U, dims, d, gamma, bc, floors, 4)
@@ -806,11 +817,17 @@ def main():
                              "binary128", "io256"])
     ap.add_argument("--n", type=int, default=None, help="override the grid size")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--lean", action="store_true", help="skip the hipGraph replay pass "
-                    "(for profiler runs)")
+    ap.add_argument("--lean", action="store_true", help="skip the hipGraph replay pass and "
+                    "the host-array rate (for profiler runs)")
+    ap.add_argument("--shapes", action="append", default=[], metavar="PREC=X,Y,Z,U",
+                    help="use these shape ids (as a line's shape_ids reports them) for one "
+                         "precision (fp64 or split) instead of the autotuner")
     args = ap.parse_args()
     global LEAN
     LEAN = args.lean
+    for spec in args.shapes:
+        prec, ids = spec.split("=")
+        SHAPES[prec] = [int(v) for v in ids.split(",")]
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)

@@ -35,6 +35,7 @@ EXPORTS = [
     "bsn_cfl_dt_fmr3d", "bsn_fmr_run", "bsn_fmr_time", "bsn_fmr_set_mode", "bsn_fmr_autotune", "bsn_batch_profile",
     "bsn_fmr3d_step_gravity", "bsn_fmr_set_self_gravity", "bsn_solve_fmr_poisson",
     "bsn_fmr_restrict_density", "bsn_fmr_prolong_scalar", "bsn_conserved_totals_noalloc",
+    "bsn_batch_get_shapes",
 ]
 
 
@@ -107,6 +108,7 @@ def load():
         "bsn_batch_set_shapes": ([_p, _i, _i, _i], _i),
         "bsn_batch_set_mode": ([_p, _i], _i),
         "bsn_batch_set_update_shape": ([_p, _i], _i),
+        "bsn_batch_get_shapes": ([_p, C.POINTER(_i)], _i),
         "bsn_batch_step": ([_p, _pd], _i),
         "bsn_batch_run": ([_p, _d, _d, _i, _i], _i),
         "bsn_batch_times": ([_p, _pd, _pd], _i),

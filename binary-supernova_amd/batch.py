@@ -47,6 +47,12 @@ class Batch:
         runs of 16/32 cells, 2 = balanced run lengths."""
         call("bsn_batch_set_update_shape", self.handle, int(shape))
 
+    def get_shapes(self) -> list:
+        """[x, y, z flux shape, fused-kernel shape] ids in use."""
+        out = (C.c_int * 4)()
+        call("bsn_batch_get_shapes", self.handle, out)
+        return list(out)
+
     def set_mode(self, inplace: bool) -> None:
         """True: in-place step with every flux materialised; False (default):
         out-of-place step with the z sweep fused into the stage update."""

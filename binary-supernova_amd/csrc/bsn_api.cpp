@@ -1419,6 +1419,13 @@ extern "C" int bsn_batch_set_update_shape(bsn_batch* b, int shape) {
     return plan_set_zshape(*b->plan, shape);
 }
 
+extern "C" int bsn_batch_get_shapes(bsn_batch* b, int shapes[4]) {
+    if (!b || !shapes) return set_err(BSN_ERR_ARG, "NULL argument");
+    for (int d = 0; d < 3; ++d) shapes[d] = b->plan->shape[d];
+    shapes[3] = b->plan->zshape;
+    return BSN_OK;
+}
+
 extern "C" int bsn_batch_set_mode(bsn_batch* b, int inplace) {
     if (!b || (inplace != 0 && inplace != 1)) return set_err(BSN_ERR_ARG, "bad arguments");
     b->plan->inplace = inplace;

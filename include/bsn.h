@@ -249,6 +249,9 @@ int bsn_batch_set_shapes(bsn_batch* b, int sx, int sy, int sz);
  * runs of 16 or 32 cells per column tile; 2: one run length per grid chosen
  * so that the tiles fill the resident workgroups evenly. */
 int bsn_batch_set_update_shape(bsn_batch* b, int shape);
+/* The shape ids in use: x, y, z flux shapes and the fused kernel's shape (as
+ * the autotuner left them), for bsn_batch_set_shapes / _set_update_shape. */
+int bsn_batch_get_shapes(bsn_batch* b, int shapes[4]);
 /* Step form: 0 (default) = out of place, z sweep fused with the stage update;
  * 1 = in place with all fluxes materialised.  Both give the same bits. */
 int bsn_batch_set_mode(bsn_batch* b, int inplace);
