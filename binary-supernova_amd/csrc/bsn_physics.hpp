@@ -286,10 +286,13 @@ __device__ __forceinline__ float fminj(float x, float y) { return __builtin_fmin
 // The WENO-Z weights are unchanged when every beta, tau and eps is scaled by
 // one factor, so the betas here are the reference's divided by 13/12.
 //
-// A constant reconstructs to itself exactly: s and t vanish, so every
-// candidate is the constant, and the combination is written as the middle
-// candidate plus weighted differences.  A uniform state at rest then has no
-// mass or energy flux in FP32 either (an outflow boundary does not leak).
+// On a uniform state s and t vanish, every candidate is the constant
+// exactly, and the left and right values come out equal (see wenoz_pair), so
+// a uniform state at rest has no mass or energy flux in FP32 either (an
+// outflow boundary does not leak).  The combination stays a weighted average
+// of the candidates: a form around one candidate would cancel where the
+// weights single out a candidate much smaller than the others (a cold medium
+// next to a shock).
 //
 // Each value is formed by explicit operations, so a value carried from one
 // interface to the next along a walk (the indicator or candidate of a
@@ -324,33 +327,40 @@ __device__ __forceinline__ float qD(const Pos3& P, float tc, float x) {
 }
 
 // Left and right WENO5-Z (p = 2) values from the five indicators and four
-// candidate values of one interface.  One eps serves both sides: the
-// reference's 1e-36, raised to 1e-17 max(tau_L, tau_R) where needed so that
-// (tau / (eps + beta))^2 stays below 1e34 and cannot overflow FP32 (whenever
-// the raise is active the weight of that stencil is already 1 - O(1e-34));
-// the centre indicator's reciprocal is then shared.  The weights d_k are
-// scaled to (1, 6, 3) and (3, 6, 1); the normalisation removes the factor.
+// candidate values of one interface.  eps per side is the reference's 1e-36,
+// raised to 1e-17 tau where needed so that (tau / (eps + beta))^2 stays below
+// 1e34 and cannot overflow FP32 (whenever the raise is active the weight of
+// that stencil is already 1 - O(1e-34)).  One eps for both sides, raised by
+// the larger tau, would let a shock on one side swamp the tiny indicators of
+// a cold, nearly uniform medium on the other.  The weights d_k are scaled to
+// (1, 6, 3) and (3, 6, 1); the normalisation removes the factor.
 __device__ __forceinline__ void wenoz_pair(float bA0, float bB1, float bC2, float bA2, float bC3,
                                            float qa0, float qb1, float qr2, float qd3, float& vl,
                                            float& vr) {
     const float tauL = __builtin_fabsf(bA0 - bC2), tauR = __builtin_fabsf(bB1 - bC3);
-    const float eps = 1e-17f * __builtin_fmaxf(__builtin_fmaxf(1e-19f, tauL), tauR);
-    const float r1 = rcp(eps + bB1), r2 = rcp(eps + bA2);
+    const float epsL = 1e-17f * __builtin_fmaxf(1e-19f, tauL);
+    const float epsR = 1e-17f * __builtin_fmaxf(1e-19f, tauR);
+    // Both sides sum (outer, outer) first, then the centre, so that on a
+    // uniform state (weights 1, 6, 3 and 3, 6, 1, equal candidates) the two
+    // sums are the same operations on the same values in swapped order: the
+    // left and right values are equal and a uniform state at rest has no flux.
     {
-        const float t0 = tauL * rcp(eps + bA0), t1 = tauL * r1, t2 = tauL * rcp(eps + bC2);
+        const float t0 = tauL * rcp(epsL + bA0), t1 = tauL * rcp(epsL + bB1);
+        const float t2 = tauL * rcp(epsL + bC2);
         const float u0 = __builtin_fmaf(t0, t0, 1.0f);
         const float u1 = 6.0f * __builtin_fmaf(t1, t1, 1.0f);
         const float u2 = 3.0f * __builtin_fmaf(t2, t2, 1.0f);
-        const float num = __builtin_fmaf(u2, qr2 - qb1, u0 * (qa0 - qb1));
-        vl = __builtin_fmaf(num, rcp((u0 + u1) + u2), qb1);
+        const float num = __builtin_fmaf(u1, qb1, u0 * qa0 + u2 * qr2);
+        vl = num * rcp((u0 + u2) + u1);
     }
     {
-        const float t0 = tauR * r1, t1 = tauR * r2, t2 = tauR * rcp(eps + bC3);
+        const float t0 = tauR * rcp(epsR + bB1), t1 = tauR * rcp(epsR + bA2);
+        const float t2 = tauR * rcp(epsR + bC3);
         const float u0 = 3.0f * __builtin_fmaf(t0, t0, 1.0f);
         const float u1 = 6.0f * __builtin_fmaf(t1, t1, 1.0f);
         const float u2 = __builtin_fmaf(t2, t2, 1.0f);
-        const float num = __builtin_fmaf(u2, qd3 - qr2, u0 * (qb1 - qr2));
-        vr = __builtin_fmaf(num, rcp((u0 + u1) + u2), qr2);
+        const float num = __builtin_fmaf(u1, qr2, u0 * qb1 + u2 * qd3);
+        vr = num * rcp((u0 + u2) + u1);
     }
 }
 

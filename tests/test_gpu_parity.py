@@ -592,6 +592,37 @@ def test_split_precision_accuracy_turbulent_box():
     assert abs(m1[0] - m0[0]) / m0[0] <= 1e-12
 
 
+def test_split_precision_accuracy_sedov():
+    # A strong blast in a cold medium (p = 1e-5 next to the shocked gas): the
+    # split form on the FP64 form's dt sequence stays within 1e-7 L1(rho) of
+    # it after 60 steps at 128^3 (3.6e-9 seen).  A WENO weight form whose eps
+    # is shared between the two sides of an interface (raised by the larger
+    # tau) gave 1.7e-5 after 200 steps: the shock swamped the cold side's
+    # indicators.  (At 64^3 the blast starts within 2.5 cells and either form
+    # differs from FP64 by about 1e-5: branch switches in the rarefied
+    # centre, not this.)
+    print("split sedov 128", flush=True)
+    n, gm = 128, 5 / 3
+    dx = 1.0 / n
+    U = P.sedov_ic_3d(n, n, n, dx, dx, dx, gm, E_blast=1.0 / (gm - 1), rho_bg=1.0, P_floor=1e-5,
+                      x_offset=-0.5, y_offset=-0.5, z_offset=-0.5)
+    grids = {}
+    for prec in ("fp64", "split"):
+        g = bsn.Grid(n, n, n, prec).set_state(U)
+        g.set_params(dx, dx, dx, gm, "outflow", 1e-8, 1e-8)
+        grids[prec] = (g, bsn.Batch([g]))
+    for _ in range(60):
+        dt = bsn.cfl_dt_3d(grids["fp64"][0], n, n, n, dx, dx, dx, gm, 0.4)
+        for prec in ("fp64", "split"):
+            grids[prec][1].step([dt])
+    A = grids["fp64"][0].get_state()[NG:-NG, NG:-NG, NG:-NG, 0]
+    B = grids["split"][0].get_state()[NG:-NG, NG:-NG, NG:-NG, 0]
+    l1 = np.abs(A - B).sum() / np.abs(A).sum()
+    assert l1 <= 1e-7, l1
+    m0 = U[NG:-NG, NG:-NG, NG:-NG, 0].sum()
+    assert abs(B.sum() - m0) <= 1e-12 * m0
+
+
 def test_turbulent_box_256_conservation_and_split_error():
     # The benchmark's workload at its full size (256^3, periodic, rms Mach 2),
     # checked through size-independent properties: three steps of both forms
