@@ -148,7 +148,10 @@ struct bsn_fmr {
 // ---------------------------------------------------------------------------
 // Tile lists
 
-static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int shape) {
+// slots: workgroups the sweep kernel keeps resident (the persistent grid);
+// the y and z walks split each column into runs of one balanced length so
+// that the tile count is a whole multiple of the slots.
+static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int shape, int slots) {
     int W, H;
     flux_shape_dims(dir, shape, W, H);
     std::vector<Tile> t;
@@ -170,17 +173,35 @@ static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int 
         return t;
     }
     if (dir != 0 && flux_walk_len(dir, shape) > 0) {
-        // register walks: 64 x 4 columns, runs of L interfaces along dir
-        const int L = flux_walk_len(dir, shape);
+        // register walks: 64 x 4 columns, each split into runs along dir.
+        // The persistent launch hands every workgroup the same number of
+        // tiles, so a tile count just above a multiple of the slots leaves
+        // most workgroups idle for the last round (2304 runs of 32 over 1024
+        // slots take three rounds).  The runs of all grids therefore share
+        // one length Lt <= 1.1 * Lmax, chosen so that the column-interfaces
+        // divide evenly over m * slots runs.
+        const int Lmax = flux_walk_len(dir, shape);
+        double work = 0;    // column-interfaces
+        for (const GridDev& d : G)
+            work += (double)((d.nx + 63) / 64) * (((dir == 1 ? d.nz : d.ny) + 3) / 4) *
+                    ((dir == 1 ? d.ny : d.nz) + 1);
+        const double sl = slots > 0 ? slots : 1;
+        int m = (int)std::ceil(work / (sl * Lmax * 1.1));
+        if (m < 1) m = 1;
+        const double Lt = work / (m * sl);
         for (int g = 0; g < (int)G.size(); ++g) {
             const GridDev& d = G[g];
             const int nf = (dir == 1 ? d.ny : d.nz) + 1;      // interfaces along dir
             const int ncross = dir == 1 ? d.nz : d.ny;
+            int nr = (int)(nf / Lt + 0.5);
+            if (nr < 1) nr = 1;
+            while ((nf + nr - 1) / nr > ZT_MAX_LEN) ++nr;
             for (int c0 = 0; c0 < ncross; c0 += 4)
-                for (int f0 = 0; f0 < nf; f0 += L) {
-                    const int len = nf - f0 < L ? nf - f0 : L;
+                for (int r = 0; r < nr; ++r) {
+                    const int f0 = (int)((int64_t)nf * r / nr), f1 = (int)((int64_t)nf * (r + 1) / nr);
+                    if (f1 <= f0) continue;
                     for (int i0 = 0; i0 < d.nx; i0 += 64)
-                        t.push_back({g, i0, c0, f0 | (len << ZT_LEN_SHIFT)});
+                        t.push_back({g, i0, c0, f0 | ((f1 - f0) << ZT_LEN_SHIFT)});
                 }
         }
         return t;
@@ -280,7 +301,7 @@ static void graph_drop(hipGraphExec_t& e, hipGraph_t& g) {
 
 static int plan_set_shape(Plan& P, int dir, int shape) {
     auto G = host_descs(P);
-    auto t = flux_tiles(G, dir, shape);
+    auto t = flux_tiles(G, dir, shape, flux_slots(dir, P.prec, shape, P.ctx->cus));
     CHK(upload_vec(&P.d_ft[dir], t));
     P.nft[dir] = (int)t.size();
     P.shape[dir] = shape;

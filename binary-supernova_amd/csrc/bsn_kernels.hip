@@ -531,13 +531,19 @@ __global__ __launch_bounds__(256) void k_fwalk(const GridDev* __restrict__ grids
         if (i >= G.nx || cc >= (DIR == 1 ? G.nz : G.ny)) continue;
         T c[4][6], p[6];
         double nxt[5];
+        int npf = 0;
         // cells along the walk are read once (halo cells of neighbouring
-        // runs aside): streaming loads
+        // runs aside): streaming loads.  The load is raw and the floor is
+        // applied when the cell is placed, so that the next cell's load stays
+        // in flight during the current interface's flux.
         auto fetch = [&](int pf) {
-            if (DIR == 1) load_cell<1, true, FAST, true>(G, NG + i, pf, NG + cc, nxt);
-            else          load_cell<2, true, FAST, true>(G, NG + i, NG + cc, pf, nxt);
+            npf = pf;
+            if (DIR == 1) load_cell<1, false, FAST, true>(G, NG + i, pf, NG + cc, nxt);
+            else          load_cell<2, false, FAST, true>(G, NG + i, NG + cc, pf, nxt);
         };
         auto place = [&](int slot) {
+            if ((G.lmode & 2) && npf >= NG && npf < NG + (DIR == 1 ? G.ny : G.nz))
+                apply_floor<FAST>(nxt, G.rho_floor, G.P_floor, G.gamma);
             const double pr = Prec<T>::pressure(nxt, G.gamma);
 #pragma unroll
             for (int q = 0; q < 4; ++q) c[q][slot] = (T)nxt[q];
@@ -1562,11 +1568,15 @@ __global__ void k_fold_dt(const GridDev* __restrict__ grids, int root, int child
 // ---------------------------------------------------------------------------
 // Launch helpers
 
-static int occupancy_grid(const void* fn, int threads, size_t lds, int ntiles, int cus) {
+static int resident_blocks(const void* fn, int threads, size_t lds) {
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds) != hipSuccess || per < 1)
         per = 1;
-    int64_t g = (int64_t)per * cus;
+    return per;
+}
+
+static int occupancy_grid(const void* fn, int threads, size_t lds, int ntiles, int cus) {
+    int64_t g = (int64_t)resident_blocks(fn, threads, lds) * cus;
     if (g > ntiles) g = ntiles;
     return (int)(g < 1 ? 1 : g);
 }
@@ -1654,6 +1664,14 @@ int flux_walk_len(int dir, int shape) {
     }
     static const int lens[3] = {16, 32, 64};
     return shape >= FLUX_WALK_SHAPE0 ? lens[shape - FLUX_WALK_SHAPE0] : 0;
+}
+
+int flux_slots(int dir, int prec, int shape, int cus) {
+    // only the walks' tile lists depend on it
+    if (dir == 0 || flux_walk_len(dir, shape) <= 0) return cus;
+    const void* fn = prec == 0 ? (dir == 1 ? (const void*)&k_fwalk<1, double> : (const void*)&k_fwalk<2, double>)
+                               : (dir == 1 ? (const void*)&k_fwalk<1, float> : (const void*)&k_fwalk<2, float>);
+    return resident_blocks(fn, 256, 0) * cus;
 }
 
 int flux_shape_count(int dir) { return dir == 0 ? FLUX_XWALK_SHAPE0 + 3 : FLUX_WALK_SHAPE0 + 3; }

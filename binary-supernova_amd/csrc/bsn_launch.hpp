@@ -21,6 +21,9 @@ struct LaunchCtx {
 void flux_shape_dims(int dir, int shape, int& W, int& H);
 int flux_shape_count(int dir);
 int flux_walk_len(int dir, int shape);
+// Workgroups of a flux sweep shape that stay resident on the device (the
+// persistent launch grid before it is clipped to the tile count).
+int flux_slots(int dir, int prec, int shape, int cus);
 
 hipError_t launch_flux(const LaunchCtx& L, int dir, int prec, int shape, const GridDev* grids,
                        const Tile* tiles, int ntiles, int max_wg);
