@@ -426,15 +426,31 @@ __global__ __launch_bounds__(256) void k_fxwalk(const GridDev* __restrict__ grid
         }
         const int f0 = tl.a, j0 = tl.b, k = tl.c & ZT_K0_MASK, n = tl.c >> ZT_LEN_SHIFT;
         const int nc = n + 5;
-        for (int e = tid; e < 4 * nc; e += 256) {
-            const int a = e / nc, b = e - a * nc;
-            if (j0 + a < G.ny) {
-                double u[5];
-                load_cell<0, true, FAST>(G, f0 + b, NG + j0 + a, NG + k, u);
-                const double pr = Prec<T>::pressure(u, G.gamma);
+        {
+            // every thread issues all of its loads before it uses the first
+            // (indices clamped into the grid, so the loads need no branch),
+            // so that the tile costs one memory round trip, not NE; the floors
+            // follow the loads
+            constexpr int NE = (4 * NC + 255) / 256;
+            double u[NE][5];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) lds[q][a][at(b)] = (T)u[q];
-                lds[4][a][at(b)] = (T)pr;
+            for (int r = 0; r < NE; ++r) {
+                const int e = min(tid + 256 * r, 4 * nc - 1);
+                const int a = e / nc, b = e - a * nc;
+                load_cell<0, false, FAST>(G, f0 + b, NG + min(j0 + a, G.ny - 1), NG + k, u[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < NE; ++r) {
+                const int e = tid + 256 * r;
+                const int a = e / nc, b = e - a * nc;
+                if (e < 4 * nc && j0 + a < G.ny) {
+                    if ((G.lmode & 2) && f0 + b >= NG && f0 + b < NG + G.nx)
+                        apply_floor<FAST>(u[r], G.rho_floor, G.P_floor, G.gamma);
+                    const double pr = Prec<T>::pressure(u[r], G.gamma);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) lds[q][a][at(b)] = (T)u[r][q];
+                    lds[4][a][at(b)] = (T)pr;
+                }
             }
         }
         __syncthreads();
