@@ -45,6 +45,38 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
     return (__attribute__((address_space(1))) T*)p;
 }
 
+// base + a 32-bit byte offset.  With a uniform base (scalar registers) and a
+// per-lane offset, a load or store takes the global saddr form and needs no
+// 64-bit address arithmetic per access.  Callers keep the uniform parts (the
+// field, the z plane) in the base and only the in-plane part in the offset,
+// so the offset stays far below 4 GiB.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* at_off(T* base, uint32_t off) {
+    return (__attribute__((address_space(1))) T*)((__attribute__((address_space(1))) char*)base + off);
+}
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* at_off(const T* base, uint32_t off) {
+    return (const __attribute__((address_space(1))) T*)((const __attribute__((address_space(1))) char*)base +
+                                                          off);
+}
+
+// Load through a raw buffer descriptor: a uniform base (scalar registers, a
+// range of 2 GiB) and a 32-bit per-lane byte offset.  The compiler folds
+// a plain pointer's per-lane offset into the base and then carries that
+// 64-bit per-lane address from one loop iteration to the next, one 64-bit
+// VALU addition per load; through the descriptor the base stays scalar.
+// The base must be uniform over the wave.  NT: streaming (nontemporal).
+template <typename T, bool NT = false>
+__device__ __forceinline__ T bld(const T* base, uint32_t off) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte loads");
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc((void*)base, 0, 0x7fffffff, 0x00020000);
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, NT ? 2 : 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, NT ? 2 : 0));
+}
+
 __device__ __forceinline__ void chunk(int ntiles, int& t0, int& t1) {
     t0 = (int)(((int64_t)blockIdx.x * ntiles) / gridDim.x);
     t1 = (int)(((int64_t)(blockIdx.x + 1) * ntiles) / gridDim.x);
@@ -724,7 +756,6 @@ template <bool FAST = false>
 __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, int j, int k, int stage,
                                                      double (&d)[5], const double (&u)[5],
                                                      const double (&un)[5], int do_cfl, double dt) {
-    const int64_t c = uidx(G, NG + i, NG + j, NG + k);
     // sources in the order of the reference's coupled step
     // (scripts/run_sn50_fiducial.jl:772-781): point masses, self-gravity, sinks
     if (G.src && G.src->nbh > 0) bh_gravity_add(*G.src, i, j, k, u, d);
@@ -770,8 +801,11 @@ __device__ __forceinline__ uint64_t finish_cell_with(const GridDev& G, int i, in
     // streaming stores: the new state is not read again by this kernel
     // (measured 1.5% faster at 256^3 than cached stores; nontemporal loads of
     // the x and y fluxes were 7% slower and are not used)
+    // plane NG + k as the base, the cell's place in it as the offset
+    double* const ob = G.Uout + (int64_t)(NG + k) * G.Ny * G.pitch;
+    const uint32_t oo = (uint32_t)uidx(G, NG + i, NG + j, 0) * (uint32_t)sizeof(double);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) __builtin_nontemporal_store(v[q], &gmem(G.Uout)[q * G.fs + c]);
+    for (int q = 0; q < 5; ++q) __builtin_nontemporal_store(v[q], at_off(ob + q * G.fs, oo));
     return do_cfl ? cfl_speed_bits<FAST>(v, G) : 0;
 }
 
@@ -869,12 +903,15 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
         if (i < G.nx && j < G.ny) {
             const int kend = min(k0 + klen, G.nz);
             const double dt = G.scal[SCAL_DT];
-            const auto Fx = gmem((const T*)G.F[0]);
-            const auto Fy = gmem((const T*)G.F[1]);
             T c[4][6], p[6];
             // interface f (between active cells f-1 and f) uses padded z = f .. f+5
             const int64_t col = uidx(G, NG + i, NG + j, 0);
             const int64_t zs = (int64_t)G.Ny * G.pitch;
+            // every access below is a uniform base (field, z plane) plus the
+            // lane's byte offset in the plane
+            const uint32_t uo = (uint32_t)col * (uint32_t)sizeof(double);
+            const uint32_t fxo = (uint32_t)(j * G.fp[0] + i) * (uint32_t)sizeof(T);
+            const uint32_t fyo = (uint32_t)(j * G.fp[1] + i) * (uint32_t)sizeof(T);
             double nxt[5];
             int npz = 0;
             // the z window and the base state are read once per launch:
@@ -883,7 +920,14 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
             // rows share, stay cached)
             auto fetch = [&](int pz) {
                 npz = pz;
-                load_cell<2, !RING, FAST, true>(G, NG + i, NG + j, pz, nxt);
+                // load_cell along z: a ghost plane reads the active plane its
+                // index maps to; without the ring an active cell is floored here
+                const int sk = (G.lmode & 1) ? bc_src(pz, G.nz, G.bc & 4) : pz;
+                const double* const b = G.U + sk * zs;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) nxt[q] = bld<double, true>(b + q * G.fs, uo);
+                if (!RING && (G.lmode & 2) && pz >= NG && pz < NG + G.nz)
+                    apply_floor<FAST>(nxt, G.rho_floor, G.P_floor, G.gamma);
             };
             auto place = [&](int slot) {
                 if (RING) {
@@ -903,25 +947,26 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
             double u[5], un[5];
             // the operands of the update of active cell k
             auto operands = [&](int k) {
-                const int64_t ox = ((int64_t)k * G.ny + j) * G.fp[0] + i;
-                const int64_t oy = ((int64_t)k * (G.ny + 1) + j) * G.fp[1] + i;
+                const T* const xb = (const T*)G.F[0] + (int64_t)k * G.ny * G.fp[0];
+                const T* const yb = (const T*)G.F[1] + (int64_t)k * (G.ny + 1) * G.fp[1];
 #pragma unroll
                 for (int q = 0; q < 5; ++q) {
-                    fx[0][q] = Fx[q * G.ffs[0] + ox];
-                    fx[1][q] = Fx[q * G.ffs[0] + ox + 1];
-                    fy[0][q] = Fy[q * G.ffs[1] + oy];
-                    fy[1][q] = Fy[q * G.ffs[1] + oy + G.fp[1]];
+                    fx[0][q] = bld(xb + q * G.ffs[0], fxo);
+                    fx[1][q] = bld(xb + q * G.ffs[0] + 1, fxo);
+                    fy[0][q] = bld(yb + q * G.ffs[1], fyo);
+                    fy[1][q] = bld(yb + q * G.ffs[1] + G.fp[1], fyo);
                 }
                 const int64_t cc = col + (int64_t)(NG + k) * zs;
                 if (RING) {
                     const int sl = (NG + k) & 3;
                     const bool un_in_ring = G.Un == G.U;
                     const bool need_u = stage > 1 || G.phi != nullptr || G.src != nullptr;
+                    const double* const nb = G.Un + (NG + k) * zs;
 #pragma unroll
                     for (int q = 0; q < 5; ++q) {
                         const double r = ring[sl][q][rt];
                         u[q] = r;
-                        un[q] = un_in_ring ? r : __builtin_nontemporal_load(&gmem(G.Un)[q * G.fs + cc]);
+                        un[q] = un_in_ring ? r : bld<double, true>(nb + q * G.fs, uo);
                     }
                     if (need_u && (G.lmode & 2)) apply_floor<FAST>(u, G.rho_floor, G.P_floor, G.gamma);
                 } else {
