@@ -188,7 +188,9 @@ static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int 
         const double sl = slots > 0 ? slots : 1;
         int m = (int)std::ceil(work / (sl * Lmax * 1.1));
         if (m < 1) m = 1;
-        const double Lt = work / (m * sl);
+        // a batch too small to fill the slots keeps runs of at least 8:
+        // shorter runs would only add window starts
+        const double Lt = std::max(work / (m * sl), 8.0);
         for (int g = 0; g < (int)G.size(); ++g) {
             const GridDev& d = G[g];
             const int nf = (dir == 1 ? d.ny : d.nz) + 1;      // interfaces along dir

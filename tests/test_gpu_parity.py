@@ -413,6 +413,38 @@ def test_batch_matches_single_bitwise():
         np.testing.assert_array_equal(g.get_state(), U)
 
 
+@pytest.mark.parametrize("inplace", [False, True])
+def test_walk_shapes_on_an_uneven_batch_bitwise(inplace):
+    # The walks' tile lists on a batch of uneven grids: every x walk and
+    # every y (and, in place, z) walk of balanced runs, FP64, against the
+    # oracle bit for bit.  The runs of all grids share one length chosen from
+    # the resident workgroups, so these grids get runs of different counts
+    # and a tiny batch keeps runs of 8 or more.
+    print(f"walk shapes on an uneven batch, inplace={inplace}", flush=True)
+    shapes = [(32, 48, 16), (17, 9, 33), (64, 8, 8), (70, 130, 20)]
+    states = [random_state(*s, seed=40 + i, wild=False) for i, s in enumerate(shapes)]
+    dts = [1e-3, 2e-3, 5e-4, 1e-3]
+    ref = []
+    for U, dt in zip(states, dts):
+        V = U.copy()
+        O.euler3d_step(V, 0.05, 0.04, 0.03, dt, 1.4, "periodic", 1e-6, 1e-6)
+        ref.append(V)
+    nwx, nwy = 3, 3                       # walk shapes: x ids 3..5, y/z ids 5..7
+    for sx in range(3, 3 + nwx):
+        for sy in range(5, 5 + nwy):
+            grids = []
+            for s, U in zip(shapes, states):
+                g = bsn.Grid(*s).set_state(U)
+                g.set_params(0.05, 0.04, 0.03, 1.4, "periodic", 1e-6, 1e-6)
+                grids.append(g)
+            b = bsn.Batch(grids)
+            b.set_mode(inplace)
+            b.set_shapes(sx, sy, sy)
+            b.step(dts)
+            for n, (g, V) in enumerate(zip(grids, ref)):
+                np.testing.assert_array_equal(g.get_state(), V, err_msg=f"grid {n} shapes {sx} {sy}")
+
+
 def test_batch_of_48_uneven_problems():
     # The batch workload itself: 48 Sod / Sedov problems of uneven sizes
     # (batch_problems(48, seed=4321), 18.2 M cells) in one Batch, autotuned,
