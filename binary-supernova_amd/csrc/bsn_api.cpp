@@ -148,6 +148,24 @@ struct bsn_fmr {
 // ---------------------------------------------------------------------------
 // Tile lists
 
+// Column tiles of a grid's (x, cross) plane for the y/z walks and the fused
+// kernel: workgroups of 64 x 4 (x, rows) and, where a grid's rows end 32
+// cells or fewer into their last x-tile, half-width workgroups of 32 x 8 for
+// that tile (COL_HALF), so that the lanes past the row end do not idle.
+static int col_tile_count(int nx, int ncross) {
+    const int rem = nx % 64, nfull = nx / 64 + (rem > 32 ? 1 : 0);
+    return nfull * ((ncross + 3) / 4) + (rem > 0 && rem <= 32 ? (ncross + 7) / 8 : 0);
+}
+
+template <typename F>
+static void col_tiles(int nx, int ncross, F&& emit) {
+    const int rem = nx % 64, nfull = nx / 64 + (rem > 32 ? 1 : 0);
+    for (int b = 0; b < ncross; b += 4)
+        for (int t = 0; t < nfull; ++t) emit(64 * t, b);
+    if (rem > 0 && rem <= 32)
+        for (int b = 0; b < ncross; b += 8) emit((64 * (nx / 64)) | COL_HALF, b);
+}
+
 // slots: workgroups the sweep kernel keeps resident (the persistent grid);
 // the y and z walks split each column into runs of one balanced length so
 // that the tile count is a whole multiple of the slots.
@@ -183,7 +201,7 @@ static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int 
         const int Lmax = flux_walk_len(dir, shape);
         double work = 0;    // column-interfaces
         for (const GridDev& d : G)
-            work += (double)((d.nx + 63) / 64) * (((dir == 1 ? d.nz : d.ny) + 3) / 4) *
+            work += (double)col_tile_count(d.nx, dir == 1 ? d.nz : d.ny) *
                     ((dir == 1 ? d.ny : d.nz) + 1);
         const double sl = slots > 0 ? slots : 1;
         int m = (int)std::ceil(work / (sl * Lmax * 1.1));
@@ -198,13 +216,13 @@ static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int 
             int nr = (int)(nf / Lt + 0.5);
             if (nr < 1) nr = 1;
             while ((nf + nr - 1) / nr > ZT_MAX_LEN) ++nr;
-            for (int c0 = 0; c0 < ncross; c0 += 4)
-                for (int r = 0; r < nr; ++r) {
-                    const int f0 = (int)((int64_t)nf * r / nr), f1 = (int)((int64_t)nf * (r + 1) / nr);
-                    if (f1 <= f0) continue;
-                    for (int i0 = 0; i0 < d.nx; i0 += 64)
-                        t.push_back({g, i0, c0, f0 | ((f1 - f0) << ZT_LEN_SHIFT)});
-                }
+            for (int r = 0; r < nr; ++r) {
+                const int f0 = (int)((int64_t)nf * r / nr), f1 = (int)((int64_t)nf * (r + 1) / nr);
+                if (f1 <= f0) continue;
+                col_tiles(d.nx, ncross, [&](int a, int b) {
+                    t.push_back({g, a, b, f0 | ((f1 - f0) << ZT_LEN_SHIFT)});
+                });
+            }
         }
         return t;
     }
@@ -252,8 +270,7 @@ static std::vector<Tile> zupdate_tiles(const std::vector<GridDev>& G, int seg, i
     std::vector<int> nseg(G.size(), 1);
     if (seg == 2) {
         double work = 0;    // column-cells
-        for (const GridDev& d : G)
-            work += (double)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz;
+        for (const GridDev& d : G) work += (double)col_tile_count(d.nx, d.ny) * d.nz;
         const double s = slots > 0 ? slots : 1;
         int m = 1;
         while (work / (m * s) > 96.0) ++m;
@@ -274,8 +291,7 @@ static std::vector<Tile> zupdate_tiles(const std::vector<GridDev>& G, int seg, i
             const int k0 = (int)((int64_t)d.nz * s / n), k1 = (int)((int64_t)d.nz * (s + 1) / n);
             if (k1 <= k0) continue;
             const int c = k0 | ((k1 - k0) << ZT_LEN_SHIFT);
-            for (int j0 = 0; j0 < d.ny; j0 += 4)
-                for (int i0 = 0; i0 < d.nx; i0 += 64) t.push_back({g, i0, j0, c});
+            col_tiles(d.nx, d.ny, [&](int a, int b) { t.push_back({g, a, b, c}); });
         }
     }
     return t;

@@ -82,6 +82,11 @@ struct Tile {
     int g, a, b, c;
 };
 
+// Column tiles (the y/z walks and the fused z-sweep + update) flag in a a
+// half-width tile: 32 lanes along x and 8 rows per workgroup instead of 64 and
+// 4, for the last x-tile of a grid whose rows end 32 cells or fewer into it.
+#define COL_HALF (1 << 30)
+
 // Fused z-sweep + update tiles pack the first cell and the column length in c.
 #define ZT_K0_MASK 0x1FFFFF
 #define ZT_LEN_SHIFT 21

@@ -82,6 +82,15 @@ __device__ __forceinline__ void chunk(int ntiles, int& t0, int& t1) {
     t1 = (int)(((int64_t)(blockIdx.x + 1) * ntiles) / gridDim.x);
 }
 
+// Lane -> (x, row) of a column tile: 64 lanes along x and one row per wave,
+// or, for a half-width tile (COL_HALF in a), 32 lanes along x and two rows per
+// wave.  Uniform over the workgroup.
+__device__ __forceinline__ void col_lane(int a, int b, int& i, int& row) {
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    if (a & COL_HALF) { i = (a & ~COL_HALF) + (lx & 31); row = b + 2 * ly + (lx >> 5); }
+    else              { i = a + lx; row = b + ly; }
+}
+
 __device__ __forceinline__ int find_grid(const int64_t* __restrict__ off, int ngrids, int64_t e) {
     int lo = 0, hi = ngrids - 1;
     while (lo < hi) {
@@ -570,12 +579,12 @@ __global__ __launch_bounds__(256) void k_fwalk(const GridDev* __restrict__ grids
     constexpr bool FAST = sizeof(T) == 4;        // split form
     int t0, t1;
     chunk(ntiles, t0, t1);
-    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     for (int t = t0; t < t1; ++t) {
         const Tile tl = tiles[t];
         const GridDev& G = grids[tl.g];
         const int f0 = tl.c & ZT_K0_MASK, len = tl.c >> ZT_LEN_SHIFT;
-        const int i = tl.a + lx, cc = tl.b + ly;
+        int i, cc;
+        col_lane(tl.a, tl.b, i, cc);
         if (i >= G.nx || cc >= (DIR == 1 ? G.nz : G.ny)) continue;
         T c[4][6], p[6];
         double nxt[5];
@@ -892,13 +901,13 @@ __global__ __launch_bounds__(256, WB) void k_zupdate(const GridDev* __restrict__
     const int rt = RING ? (int)threadIdx.x : 0;
     int t0, t1;
     chunk(ntiles, t0, t1);
-    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     for (int t = t0; t < t1; ++t) {
         const Tile tl = tiles[t];
         const GridDev& G = grids[tl.g];
         // tile.c packs the first cell k0 (bits 0-20) and the column length (bits 21-30)
         const int k0 = tl.c & ZT_K0_MASK, klen = tl.c >> ZT_LEN_SHIFT;
-        const int i = tl.a + lx, j = tl.b + ly;
+        int i, j;
+        col_lane(tl.a, tl.b, i, j);
         uint64_t smax = 0;
         if (i < G.nx && j < G.ny) {
             const int kend = min(k0 + klen, G.nz);
