@@ -46,18 +46,13 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
 }
 
 // base + a 32-bit byte offset.  With a uniform base (scalar registers) and a
-// per-lane offset, a load or store takes the global saddr form and needs no
-// 64-bit address arithmetic per access.  Callers keep the uniform parts (the
-// field, the z plane) in the base and only the in-plane part in the offset,
-// so the offset stays far below 4 GiB.
+// per-lane offset, a store takes the global saddr form and needs no 64-bit
+// address arithmetic.  Callers keep the uniform parts (the field, the z plane)
+// in the base and only the in-plane part in the offset, so the offset stays
+// far below 4 GiB.
 template <typename T>
 __device__ __forceinline__ __attribute__((address_space(1))) T* at_off(T* base, uint32_t off) {
     return (__attribute__((address_space(1))) T*)((__attribute__((address_space(1))) char*)base + off);
-}
-template <typename T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T* at_off(const T* base, uint32_t off) {
-    return (const __attribute__((address_space(1))) T*)((const __attribute__((address_space(1))) char*)base +
-                                                          off);
 }
 
 // Load through a raw buffer descriptor: a uniform base (scalar registers, a
