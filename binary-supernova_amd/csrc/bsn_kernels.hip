@@ -607,7 +607,7 @@ __global__ __launch_bounds__(256) void k_fwalk(const GridDev* __restrict__ grids
         sp::WenoCarry wc[5];
         if constexpr (FAST) sp::interface_walk_start(c, p, wc);
         const int fend = f0 + len;
-        for (int f = f0; f < fend; ++f) {
+        auto step = [&](int f) {
             place(5);                                   // cell f+5 (padded)
             if (f + 1 < fend) fetch(f + 6);             // in flight during the flux
             T F[5];
@@ -622,7 +622,19 @@ __global__ __launch_bounds__(256) void k_fwalk(const GridDev* __restrict__ grids
                 for (int q = 0; q < 4; ++q) c[q][m] = c[q][m + 1];
                 p[m] = p[m + 1];
             }
+        };
+        int f = f0;
+        if constexpr (FAST) {
+            // six interfaces per pass: the window's slots come back to the
+            // same registers after six shifts, so the shifts cost no moves
+            // (split form: 754 -> 695 VALU instructions per interface, 144
+            // VGPRs; the FP64 form would double its registers)
+            for (; f + 6 <= fend; f += 6) {
+#pragma unroll
+                for (int u = 0; u < 6; ++u) step(f + u);
+            }
         }
+        for (; f < fend; ++f) step(f);
     }
 }
 
