@@ -191,7 +191,8 @@ static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int 
         return t;
     }
     if (dir != 0 && flux_walk_len(dir, shape) > 0) {
-        // register walks: 64 x 4 columns, each split into runs along dir.
+        // register walks: column tiles (col_tiles), each column split into
+        // runs along dir.
         // The persistent launch hands every workgroup the same number of
         // tiles, so a tile count just above a multiple of the slots leaves
         // most workgroups idle for the last round (2304 runs of 32 over 1024
