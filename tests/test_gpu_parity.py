@@ -413,21 +413,22 @@ def test_batch_matches_single_bitwise():
         np.testing.assert_array_equal(g.get_state(), U)
 
 
-@pytest.mark.parametrize("inplace", [False, True])
-def test_walk_shapes_on_an_uneven_batch_bitwise(inplace):
+@pytest.mark.parametrize("inplace,bc", [(False, "periodic"), (True, "periodic"), (False, "outflow")])
+def test_walk_shapes_on_an_uneven_batch_bitwise(inplace, bc):
     # The walks' tile lists on a batch of uneven grids: every x walk and
     # every y (and, in place, z) walk of balanced runs, FP64, against the
     # oracle bit for bit.  The runs of all grids share one length chosen from
     # the resident workgroups, so these grids get runs of different counts
-    # and a tiny batch keeps runs of 8 or more.
-    print(f"walk shapes on an uneven batch, inplace={inplace}", flush=True)
+    # and a tiny batch keeps runs of 8 or more.  Rows that end 32 cells or
+    # fewer into their last x-tile (nx = 17, 32, 70) run as half-width tiles.
+    print(f"walk shapes on an uneven batch, inplace={inplace}, bc={bc}", flush=True)
     shapes = [(32, 48, 16), (17, 9, 33), (64, 8, 8), (70, 130, 20)]
     states = [random_state(*s, seed=40 + i, wild=False) for i, s in enumerate(shapes)]
     dts = [1e-3, 2e-3, 5e-4, 1e-3]
     ref = []
     for U, dt in zip(states, dts):
         V = U.copy()
-        O.euler3d_step(V, 0.05, 0.04, 0.03, dt, 1.4, "periodic", 1e-6, 1e-6)
+        O.euler3d_step(V, 0.05, 0.04, 0.03, dt, 1.4, bc, 1e-6, 1e-6)
         ref.append(V)
     nwx, nwy = 3, 3                       # walk shapes: x ids 3..5, y/z ids 5..7
     for sx in range(3, 3 + nwx):
@@ -435,7 +436,7 @@ def test_walk_shapes_on_an_uneven_batch_bitwise(inplace):
             grids = []
             for s, U in zip(shapes, states):
                 g = bsn.Grid(*s).set_state(U)
-                g.set_params(0.05, 0.04, 0.03, 1.4, "periodic", 1e-6, 1e-6)
+                g.set_params(0.05, 0.04, 0.03, 1.4, bc, 1e-6, 1e-6)
                 grids.append(g)
             b = bsn.Batch(grids)
             b.set_mode(inplace)
