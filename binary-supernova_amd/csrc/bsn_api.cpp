@@ -195,8 +195,8 @@ static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int 
         // runs along dir.
         // The persistent launch hands every workgroup the same number of
         // tiles, so a tile count just above a multiple of the slots leaves
-        // most workgroups idle for the last round (2304 runs of 32 over 1024
-        // slots take three rounds).  The runs of all grids therefore share
+        // most workgroups idle for a final pass (2304 runs of 32 over 1024
+        // slots take three passes).  The runs of all grids therefore share
         // one length Lt <= 1.1 * Lmax, chosen so that the column-interfaces
         // divide evenly over m * slots runs.
         const int Lmax = flux_walk_len(dir, shape);
