@@ -175,7 +175,7 @@ def roofline(res, prec, interfaces_per_sweep, cells, n=None):
     flux_ms = sum(fl) / len(fl)
     upd_ms = res["ms_update"]
     peak = VALU_FP64_TFLOPS if prec == "fp64" else VALU_FP32_TFLOPS
-    flux = dict(kernel="k_flux (x and y sweeps, avg)", bound="valu",
+    flux = dict(kernel="x and y flux sweeps (k_fxwalk, k_fwalk or k_flux as tuned), avg", bound="valu",
                 achieved=FLOPS_PER_INTERFACE * interfaces_per_sweep / (flux_ms * 1e-3) / 1e12,
                 peak=peak, unit="TFLOP/s", traffic=None)
     flux["frac"] = flux["achieved"] / peak
