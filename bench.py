@@ -170,7 +170,7 @@ def roofline(res, prec, interfaces_per_sweep, cells, n=None):
     """Roofline positions.  The dominant kernel is the fused z-sweep +
     stage-update kernel (k_zupdate, the largest share of the step): HBM-bound
     by its algorithmic bytes, with the VALU rate of its z-flux work beside
-    it.  The x and y flux sweeps (k_flux) are VALU-bound."""
+    it.  The x and y flux sweeps are VALU-bound."""
     fl = [m for m in res["ms_flux"] if m > 0]
     flux_ms = sum(fl) / len(fl)
     upd_ms = res["ms_update"]
