@@ -433,12 +433,13 @@ static int plan_new(bsn_ctx* ctx, const std::vector<bsn_grid*>& grids, Plan** ou
         if (g->h.prec != P->prec) { delete P; return set_err(BSN_ERR_ARG, "grids of mixed precision"); }
         if (g->ctx != ctx) { delete P; return set_err(BSN_ERR_ARG, "grids of different contexts"); }
     }
-    // starting shapes (kept until bsn_batch_autotune or set_*shape*): the
-    // ones the autotuner keeps for the 256^3 box on MI355X.  Split: x 64x4,
-    // y and z 32x8, the fused kernel with the LDS ring, the early load and
-    // balanced runs; FP64: x 16x16, y and z 32x8, balanced runs.
-    if (P->prec == 1) { P->shape[0] = 0; P->shape[1] = 2; P->shape[2] = 2; P->zshape = 11; }
-    else              { P->shape[0] = 2; P->shape[1] = 2; P->shape[2] = 2; P->zshape = 2; }
+    // starting shapes (kept until bsn_batch_autotune or set_*shape*): y and z
+    // register walks in balanced runs (the autotuner's choice from 32^3
+    // batches to the 256^3 box), the fused kernel with the LDS ring, the
+    // early load and balanced runs; the x sweep tiled (64x4 split, 16x16
+    // FP64), which unlike the x walks keeps its lanes busy on narrow grids.
+    if (P->prec == 1) { P->shape[0] = 0; P->shape[1] = FLUX_WALK_SHAPE0 + 2; P->shape[2] = FLUX_WALK_SHAPE0 + 2; P->zshape = 11; }
+    else              { P->shape[0] = 2; P->shape[1] = FLUX_WALK_SHAPE0 + 2; P->shape[2] = FLUX_WALK_SHAPE0 + 2; P->zshape = 11; }
     int r = plan_build(*P);
     if (r != BSN_OK) { plan_free(P); return r; }
     *out = P;
