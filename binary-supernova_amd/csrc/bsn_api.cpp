@@ -612,6 +612,14 @@ extern "C" int bsn_grid_create(bsn_ctx* ctx, int64_t nx, int64_t ny, int64_t nz,
                        (long long)ny, (long long)nz);
     if (precision != BSN_PREC_FP64 && precision != BSN_PREC_SPLIT)
         return set_err(BSN_ERR_ARG, "precision %d", precision);
+    // the kernels address one z plane of a field (state or x/y faces)
+    // through a 32-bit byte offset from the plane's base, below 2 GiB
+    {
+        const int64_t plane = (int64_t)(ny + 2 * BSN_NG + 1) * (BSN_XOFF + nx + 2 * BSN_NG + 16) * 8;
+        if (plane >= ((int64_t)1 << 31))
+            return set_err(BSN_ERR_ARG, "an x-y plane of %lld x %lld cells exceeds 2 GiB per field",
+                           (long long)nx, (long long)ny);
+    }
     HIPCHK(hipSetDevice(ctx->device));
     bsn_grid* g = new bsn_grid();
     g->ctx = ctx;

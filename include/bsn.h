@@ -61,7 +61,9 @@ int bsn_ctx_info(bsn_ctx* ctx, char* buf, int len);
 
 /* A uniform grid (one problem or one FMR level) resident on the device.
  * Replaces the (U, nx, ny, nz) argument group of the reference's functions
- * and the FMRLevel3D struct (src/fmr3d.jl:57-71). */
+ * and the FMRLevel3D struct (src/fmr3d.jl:57-71).  One x-y plane of one
+ * field, ghosts included, must stay under 2 GiB (nx, ny up to about 16000):
+ * the kernels address a plane through 32-bit offsets; BSN_ERR_ARG otherwise. */
 int bsn_grid_create(bsn_ctx* ctx, int64_t nx, int64_t ny, int64_t nz, int precision,
                     bsn_grid** out);
 int bsn_grid_destroy(bsn_grid* g);

@@ -396,6 +396,16 @@ def test_fmr_full_size_forms_agree():
         np.testing.assert_array_equal(a, b)
 
 
+def test_grid_with_a_plane_over_2_gib_is_refused():
+    # One x-y plane of a field must stay under 2 GiB (32-bit in-plane
+    # offsets): refused before anything is allocated.
+    print("plane over 2 GiB", flush=True)
+    with pytest.raises(bsn.BsnError, match="exceeds 2 GiB"):
+        bsn.Grid(17000, 17000, 1)
+    g = bsn.Grid(64, 64, 8)
+    assert g.nx == 64
+
+
 def test_batch_matches_single_bitwise():
     print("batch", flush=True)
     shapes = [(32, 48, 16), (17, 9, 33), (64, 8, 8)]
