@@ -275,7 +275,8 @@ __global__ void k_floors(const GridDev* __restrict__ grids, const int64_t* __res
 //   DIR 0 (x): tile (g, f0, j0, k): interfaces f0..f0+W-1 (f < nx) along x
 //              for rows j0..j0+H-1.  Lanes run along x.  A tile with
 //              a = -1 instead computes the last interface f = nx for the
-//              rows j0..j0+W*H-1, gathering from HBM.
+//              rows j0..j0+W*H-1, gathering from HBM.  With W = 64 each wave
+//              stages and computes its own row, with no block barrier.
 //   DIR 1 (y): tile (g, f0, i0, k): interfaces f0..f0+H-1 (f <= ny) along y
 //              for cells i0..i0+W-1.  Lanes run along x (coalesced).
 //   DIR 2 (z): tile (g, f0, i0, j): as DIR 1 along z.
@@ -324,6 +325,9 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
     constexpr bool FAST = sizeof(T) == 4;        // split form
     constexpr int SA = DIR == 0 ? H : H + 5;     // LDS rows
     constexpr int SB = DIR == 0 ? W + 5 : W;     // LDS row length
+    // x tiles one wave wide: each wave stages and computes its own row,
+    // with no block barrier
+    constexpr bool WAVE = DIR == 0 && W == 64;
     __shared__ T lds[5][SA][SB];
     const int tid = threadIdx.x;
     int t0, t1;
@@ -359,6 +363,51 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
                 }
                 emit(G, c, p, f, ci, cj);
             }
+            continue;
+        }
+        if constexpr (WAVE) {
+            // each wave stages and computes its own row h of the tile: cells
+            // f0 .. f0+68 of row j0+h, both loads issued before either is used
+            // (indices clamped into the grid), then floors and pressures
+            const int h = tid >> 6, x = tid & 63;
+            const bool row_ok = tl.b + h < G.ny;
+            const int pj = NG + min(tl.b + h, G.ny - 1), pk = NG + tl.c;
+            double u[2][5];
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+                load_cell<0, false, FAST>(G, min(tl.a + (r ? 64 + min(x, 4) : x), Nx - 1), pj, pk, u[r]);
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int b = r ? 64 + x : x, pi = tl.a + b;
+                if ((r == 0 || x < 5) && row_ok && pi < Nx) {
+                    if ((G.lmode & 2) && pi >= NG && pi < NG + G.nx)
+                        apply_floor<FAST>(u[r], G.rho_floor, G.P_floor, G.gamma);
+                    const double pr = Prec<T>::pressure(u[r], G.gamma);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) lds[q][h][b] = (T)u[r][q];
+                    lds[4][h][b] = (T)pr;
+                }
+            }
+            // the row's cells come from the other lanes of this wave: the LDS
+            // runs one wave's operations in order, and the fences keep the
+            // compiler from moving the reads above the writes
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int f = tl.a + x, ci = tl.b + h;
+            if (f < G.nx && ci < G.ny) {
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) c[q][m] = lds[q][h][x + m];
+                    p[m] = lds[4][h][x + m];
+                }
+                emit(G, c, p, f, ci, tl.c);
+            }
+            // the next tile's stores follow this tile's reads (same order)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             continue;
         }
         // stage cells into LDS
