@@ -275,8 +275,9 @@ __global__ void k_floors(const GridDev* __restrict__ grids, const int64_t* __res
 //   DIR 0 (x): tile (g, f0, j0, k): interfaces f0..f0+W-1 (f < nx) along x
 //              for rows j0..j0+H-1.  Lanes run along x.  A tile with
 //              a = -1 instead computes the last interface f = nx for the
-//              rows j0..j0+W*H-1, gathering from HBM.  With W = 64 each wave
-//              stages and computes its own row, with no block barrier.
+//              rows j0..j0+W*H-1, gathering from HBM.  In the 64 x 4 tile
+//              each wave stages and computes its own row, with no block
+//              barrier.
 //   DIR 1 (y): tile (g, f0, i0, k): interfaces f0..f0+H-1 (f <= ny) along y
 //              for cells i0..i0+W-1.  Lanes run along x (coalesced).
 //   DIR 2 (z): tile (g, f0, i0, j): as DIR 1 along z.
@@ -325,9 +326,9 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
     constexpr bool FAST = sizeof(T) == 4;        // split form
     constexpr int SA = DIR == 0 ? H : H + 5;     // LDS rows
     constexpr int SB = DIR == 0 ? W + 5 : W;     // LDS row length
-    // x tiles one wave wide: each wave stages and computes its own row,
-    // with no block barrier
-    constexpr bool WAVE = DIR == 0 && W == 64;
+    // x tiles 64 x 4: each wave stages and computes its own row, with no
+    // block barrier
+    constexpr bool WAVE = DIR == 0 && W == 64 && H == 4;
     __shared__ T lds[5][SA][SB];
     const int tid = threadIdx.x;
     int t0, t1;
