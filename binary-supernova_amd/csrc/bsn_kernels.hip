@@ -370,13 +370,20 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
             // each wave stages and computes its own row h of the tile: cells
             // f0 .. f0+68 of row j0+h, both loads issued before either is used
             // (indices clamped into the grid), then floors and pressures
-            const int h = tid >> 6, x = tid & 63;
+            // (the row is uniform over the wave: its base stays in scalar
+            // registers and the loads and stores take 32-bit lane offsets)
+            const int h = __builtin_amdgcn_readfirstlane(tid >> 6), x = tid & 63;
             const bool row_ok = tl.b + h < G.ny;
             const int pj = NG + min(tl.b + h, G.ny - 1), pk = NG + tl.c;
+            const double* rowU = G.U + uidx(G, 0, pj, pk);
             double u[2][5];
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
-                load_cell<0, false, FAST>(G, min(tl.a + (r ? 64 + min(x, 4) : x), Nx - 1), pj, pk, u[r]);
+            for (int r = 0; r < 2; ++r) {
+                int pi = min(tl.a + (r ? 64 + min(x, 4) : x), Nx - 1);
+                if (G.lmode & 1) pi = bc_src(pi, G.nx, G.bc & 1);
+#pragma unroll
+                for (int q = 0; q < 5; ++q) u[r][q] = bld(rowU + q * G.fs, (uint32_t)pi * 8u);
+            }
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 const int b = r ? 64 + x : x, pi = tl.a + b;
@@ -403,7 +410,12 @@ __global__ __launch_bounds__(W * H) void k_flux(const GridDev* __restrict__ grid
                     for (int q = 0; q < 4; ++q) c[q][m] = lds[q][h][x + m];
                     p[m] = lds[4][h][x + m];
                 }
-                emit(G, c, p, f, ci, tl.c);
+                T F[5];
+                Prec<T>::template flux<0>(c, p, G, F);
+                T* rowF = (T*)G.F[0] + ((int64_t)tl.c * G.ny + ci) * G.fp[0];
+#pragma unroll
+                for (int q = 0; q < 5; ++q)
+                    __builtin_nontemporal_store(F[q], at_off(rowF + q * G.ffs[0], (uint32_t)f * (uint32_t)sizeof(T)));
             }
             // the next tile's stores follow this tile's reads (same order)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -630,18 +642,33 @@ __global__ __launch_bounds__(256) void k_fwalk(const GridDev* __restrict__ grids
         const int f0 = tl.c & ZT_K0_MASK, len = tl.c >> ZT_LEN_SHIFT;
         int i, cc;
         col_lane(tl.a, tl.b, i, cc);
+        // the wave's first row (uniform) and this lane's row past it (0, or 1
+        // in the second half of a half-width tile): the walk's addresses are
+        // a scalar base per cell or face and a lane offset fixed for the tile
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int cc0 = (tl.a & COL_HALF) ? tl.b + 2 * wv : tl.b + wv;
+        const int dr = cc - cc0;
         if (i >= G.nx || cc >= (DIR == 1 ? G.nz : G.ny)) continue;
+        const uint32_t uoff =
+            (uint32_t)(((DIR == 1 ? (int64_t)G.Ny * G.pitch : (int64_t)G.pitch) * dr + NG + i) * 8);
+        const uint32_t foff =
+            (uint32_t)(((DIR == 1 ? (int64_t)(G.ny + 1) * G.fp[1] : (int64_t)G.fp[2]) * dr + i) * sizeof(T));
         T c[4][6], p[6];
         double nxt[5];
         int npf = 0;
         // cells along the walk are read once (halo cells of neighbouring
         // runs aside): streaming loads.  The load is raw and the floor is
         // applied when the cell is placed, so that the next cell's load stays
-        // in flight during the current interface's flux.
+        // in flight during the current interface's flux.  A ghost index along
+        // the walk reads the active cell it maps to, as load_cell does.
         auto fetch = [&](int pf) {
             npf = pf;
-            if (DIR == 1) load_cell<1, false, FAST, true>(G, NG + i, pf, NG + cc, nxt);
-            else          load_cell<2, false, FAST, true>(G, NG + i, NG + cc, pf, nxt);
+            int sp_ = pf;
+            if (G.lmode & 1) sp_ = bc_src(pf, DIR == 1 ? G.ny : G.nz, G.bc & (DIR == 1 ? 2 : 4));
+            const int64_t base = DIR == 1 ? ((int64_t)(NG + cc0) * G.Ny + sp_) * G.pitch + BSN_XOFF
+                                          : ((int64_t)sp_ * G.Ny + NG + cc0) * G.pitch + BSN_XOFF;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) nxt[q] = bld<double, true>(G.U + q * G.fs + base, uoff);
         };
         auto place = [&](int slot) {
             if ((G.lmode & 2) && npf >= NG && npf < NG + (DIR == 1 ? G.ny : G.nz))
@@ -665,7 +692,11 @@ __global__ __launch_bounds__(256) void k_fwalk(const GridDev* __restrict__ grids
                 sp::interface_flux<DIR, true>(c, p, G.gf32[0], G.gf32[1], G.gf32[2], F, wc);
             else
                 Prec<T>::template flux<DIR>(c, p, G, F);
-            store_face<DIR>(G, F, f, i, cc);
+            // store_face's layout, from the wave's first row
+            T* fb = (T*)G.F[DIR] + (DIR == 1 ? ((int64_t)cc0 * (G.ny + 1) + f) * G.fp[1]
+                                             : ((int64_t)f * G.ny + cc0) * G.fp[2]);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) __builtin_nontemporal_store(F[q], at_off(fb + q * G.ffs[DIR], foff));
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
 #pragma unroll
