@@ -631,7 +631,7 @@ __global__ __launch_bounds__(256) void k_fxwalk(const GridDev* __restrict__ grid
 // split form carries the smoothness indicators that consecutive interfaces
 // share.  A run of len interfaces reads len + 5 cells.
 template <int DIR, typename T>
-__global__ __launch_bounds__(256) void k_fwalk(const GridDev* __restrict__ grids,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) void k_fwalk(const GridDev* __restrict__ grids,
                                                const Tile* __restrict__ tiles, int ntiles) {
     constexpr bool FAST = sizeof(T) == 4;        // split form
     int t0, t1;
@@ -708,8 +708,10 @@ __global__ __launch_bounds__(256) void k_fwalk(const GridDev* __restrict__ grids
         if constexpr (FAST) {
             // six interfaces per pass: the window's slots come back to the
             // same registers after six shifts, so the shifts cost no moves
-            // (split form: 754 -> 695 VALU instructions per interface, 144
-            // VGPRs; the FP64 form would double its registers)
+            // (split form: 754 -> 695 VALU instructions per interface; the
+            // FP64 form would double its registers).  The split walk is held
+            // to 128 VGPRs, 4 waves per SIMD, with a few values spilled: the
+            // y sweep took 0.295 ms against 0.305 at 133 VGPRs and 3 waves.
             for (; f + 6 <= fend; f += 6) {
 #pragma unroll
                 for (int u = 0; u < 6; ++u) step(f + u);
