@@ -17,6 +17,8 @@
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <numeric>
+#include <queue>
 #include <string>
 #include <vector>
 #include "bsn_internal.hpp"
@@ -169,6 +171,55 @@ static void col_tiles(int nx, int ncross, F&& emit) {
 // slots: workgroups the sweep kernel keeps resident (the persistent grid);
 // the y and z walks split each column into runs of one balanced length so
 // that the tile count is a whole multiple of the slots.
+// Reorders a tile list of uneven tiles so that the persistent launch's
+// uniform chunks (workgroup b of G takes tiles [b n / G, (b + 1) n / G), see
+// chunk()) carry about equal work.  A batch of uneven grids gives runs of
+// different lengths, and each grid's tiles sit together, so equal tile
+// counts left some workgroups with several times the work of others.  Tiles
+// go, longest first, to the least loaded chunk that still has room (chunk
+// sizes fixed by chunk()); each chunk keeps its tiles in list order.  G is
+// the launch's grid, min(slots, n).  Lists whose tiles weigh within 10 % of
+// each other keep their spatial order.
+template <typename Weight>
+static void balance_chunks(std::vector<Tile>& t, int slots, Weight weight) {
+    const int n = (int)t.size();
+    const int G = std::min(slots, n);
+    if (G <= 1) return;
+    std::vector<double> w(n);
+    double wmin = 1e300, wmax = 0.0;
+    for (int i = 0; i < n; ++i) {
+        w[i] = weight(t[i]);
+        wmin = std::min(wmin, w[i]);
+        wmax = std::max(wmax, w[i]);
+    }
+    if (wmax <= 1.1 * wmin) return;
+    std::vector<int> cap(G);
+    for (int b = 0; b < G; ++b)
+        cap[b] = (int)(((int64_t)(b + 1) * n) / G) - (int)(((int64_t)b * n) / G);
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+    using Load = std::pair<double, int>;
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (int b = 0; b < G; ++b)
+        if (cap[b] > 0) heap.push({0.0, b});
+    std::vector<std::vector<int>> lists(G);
+    for (int i : order) {
+        const Load top = heap.top();
+        heap.pop();
+        const int b = top.second;
+        lists[b].push_back(i);
+        if ((int)lists[b].size() < cap[b]) heap.push({top.first + w[i], b});
+    }
+    std::vector<Tile> out;
+    out.reserve(n);
+    for (auto& l : lists) {
+        std::sort(l.begin(), l.end());
+        for (int i : l) out.push_back(t[i]);
+    }
+    t.swap(out);
+}
+
 static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int shape, int slots) {
     int W, H;
     flux_shape_dims(dir, shape, W, H);
@@ -225,6 +276,8 @@ static std::vector<Tile> flux_tiles(const std::vector<GridDev>& G, int dir, int 
                 });
             }
         }
+        // a run of L interfaces reads L + 5 cells
+        balance_chunks(t, slots, [](const Tile& x) { return (double)((x.c >> ZT_LEN_SHIFT) + 5); });
         return t;
     }
     for (int g = 0; g < (int)G.size(); ++g) {
@@ -295,6 +348,8 @@ static std::vector<Tile> zupdate_tiles(const std::vector<GridDev>& G, int seg, i
             col_tiles(d.nx, d.ny, [&](int a, int b) { t.push_back({g, a, b, c}); });
         }
     }
+    // a run of L cells reads a window of L + 5
+    balance_chunks(t, slots, [](const Tile& x) { return (double)((x.c >> ZT_LEN_SHIFT) + 5); });
     return t;
 }
 
