@@ -275,8 +275,8 @@ __global__ void k_floors(const GridDev* __restrict__ grids, const int64_t* __res
 //   DIR 0 (x): tile (g, f0, j0, k): interfaces f0..f0+W-1 (f < nx) along x
 //              for rows j0..j0+H-1.  Lanes run along x.  A tile with
 //              a = -1 instead computes the last interface f = nx for the
-//              rows j0..j0+W*H-1, gathering from HBM.  In the 64 x 4 tile
-//              each wave stages and computes its own row, with no block
+//              rows j0..j0+W*H-1, gathering from HBM.  Each wave stages and
+//              computes its own rows (64 / W of them), with no block
 //              barrier.
 //   DIR 1 (y): tile (g, f0, i0, k): interfaces f0..f0+H-1 (f <= ny) along y
 //              for cells i0..i0+W-1.  Lanes run along x (coalesced).
@@ -326,10 +326,11 @@ __global__ __launch_bounds__(W * H, (DIR == 0 && W == 64 && H == 4 && sizeof(T) 
     constexpr bool FAST = sizeof(T) == 4;        // split form
     constexpr int SA = DIR == 0 ? H : H + 5;     // LDS rows
     constexpr int SB = DIR == 0 ? W + 5 : W;     // LDS row length
-    // x tiles 64 x 4: each wave stages and computes its own row, with no
-    // block barrier.  The split form is held to 80 VGPRs, 6 waves per SIMD
-    // (a few values spilled): 0.367 ms against 0.376 at 91 VGPRs, 5 waves.
-    constexpr bool WAVE = DIR == 0 && W == 64 && H == 4;
+    // x tiles of 256 threads, 64, 32 or 16 wide: each wave stages and
+    // computes its own rows, with no block barrier.  The split 64 x 4 tile is
+    // held to 80 VGPRs, 6 waves per SIMD (a few values spilled): 0.367 ms
+    // against 0.376 at 91 VGPRs, 5 waves.
+    constexpr bool WAVE = DIR == 0 && W * H == 256 && (W == 64 || W == 32 || W == 16);
     __shared__ T lds[5][SA][SB];
     const int tid = threadIdx.x;
     int t0, t1;
@@ -368,26 +369,31 @@ __global__ __launch_bounds__(W * H, (DIR == 0 && W == 64 && H == 4 && sizeof(T) 
             continue;
         }
         if constexpr (WAVE) {
-            // each wave stages and computes its own row h of the tile: cells
-            // f0 .. f0+68 of row j0+h, both loads issued before either is used
-            // (indices clamped into the grid), then floors and pressures
-            // (the row is uniform over the wave: its base stays in scalar
-            // registers and the loads and stores take 32-bit lane offsets)
-            const int h = __builtin_amdgcn_readfirstlane(tid >> 6), x = tid & 63;
+            // each wave stages and computes its own RW = 64 / W rows of the
+            // tile (lane -> row hb + lane / W, x = lane % W): cells f0 .. f0+W+4
+            // of each row, both loads issued before either is used (indices
+            // clamped into the grid), then floors and pressures.  The wave's
+            // first row is uniform: its base stays in scalar registers, and a
+            // lane's row past it goes into its 32-bit load and store offsets.
+            constexpr int RW = 64 / W;
+            const int hb = __builtin_amdgcn_readfirstlane(tid >> 6) * RW, lane = tid & 63;
+            const int x = lane % W, h = hb + lane / W;
             const bool row_ok = tl.b + h < G.ny;
-            const int pj = NG + min(tl.b + h, G.ny - 1), pk = NG + tl.c;
-            const double* rowU = G.U + uidx(G, 0, pj, pk);
+            const int pjb = NG + min(tl.b + hb, G.ny - 1), pk = NG + tl.c;
+            const int pj = NG + min(tl.b + h, G.ny - 1);
+            const double* rowU = G.U + uidx(G, 0, pjb, pk);
+            const uint32_t roff = (uint32_t)((pj - pjb) * G.pitch) * 8u;
             double u[2][5];
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
-                int pi = min(tl.a + (r ? 64 + min(x, 4) : x), Nx - 1);
+                int pi = min(tl.a + (r ? W + min(x, 4) : x), Nx - 1);
                 if (G.lmode & 1) pi = bc_src(pi, G.nx, G.bc & 1);
 #pragma unroll
-                for (int q = 0; q < 5; ++q) u[r][q] = bld(rowU + q * G.fs, (uint32_t)pi * 8u);
+                for (int q = 0; q < 5; ++q) u[r][q] = bld(rowU + q * G.fs, roff + (uint32_t)pi * 8u);
             }
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
-                const int b = r ? 64 + x : x, pi = tl.a + b;
+                const int b = r ? W + x : x, pi = tl.a + b;
                 if ((r == 0 || x < 5) && row_ok && pi < Nx) {
                     if ((G.lmode & 2) && pi >= NG && pi < NG + G.nx)
                         apply_floor<FAST>(u[r], G.rho_floor, G.P_floor, G.gamma);
@@ -397,7 +403,7 @@ __global__ __launch_bounds__(W * H, (DIR == 0 && W == 64 && H == 4 && sizeof(T) 
                     lds[4][h][b] = (T)pr;
                 }
             }
-            // the row's cells come from the other lanes of this wave: the LDS
+            // the rows' cells come from other lanes of this wave: the LDS
             // runs one wave's operations in order, and the fences keep the
             // compiler from moving the reads above the writes
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -413,10 +419,11 @@ __global__ __launch_bounds__(W * H, (DIR == 0 && W == 64 && H == 4 && sizeof(T) 
                 }
                 T F[5];
                 Prec<T>::template flux<0>(c, p, G, F);
-                T* rowF = (T*)G.F[0] + ((int64_t)tl.c * G.ny + ci) * G.fp[0];
+                T* rowF = (T*)G.F[0] + ((int64_t)tl.c * G.ny + tl.b + hb) * G.fp[0];
+                const uint32_t fo = (uint32_t)((h - hb) * G.fp[0] + f) * (uint32_t)sizeof(T);
 #pragma unroll
                 for (int q = 0; q < 5; ++q)
-                    __builtin_nontemporal_store(F[q], at_off(rowF + q * G.ffs[0], (uint32_t)f * (uint32_t)sizeof(T)));
+                    __builtin_nontemporal_store(F[q], at_off(rowF + q * G.ffs[0], fo));
             }
             // the next tile's stores follow this tile's reads (same order)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
