@@ -1612,8 +1612,8 @@ static int plan_autotune(Plan& P, std::string& out) {
     // The GPU comes to the tuner idle: its clocks and TLBs settle over the
     // first few milliseconds of work, long enough to slow whichever candidate
     // is timed first by a tenth.  Whole stages run first, for at least 20 ms,
-    // and the candidates are then timed round-robin, keeping each one's best
-    // of three rounds.
+    // and the candidates are then timed in turn, keeping each one's best of
+    // three passes.
     {
         HIPCHK(hipEventRecord(ev[0], L.stream));
         for (int n = 0;; ++n) {
@@ -1626,12 +1626,12 @@ static int plan_autotune(Plan& P, std::string& out) {
             if (ms >= 20.0f || n >= 4000) break;
         }
     }
-    const int rounds = 3;
+    const int passes = 3;
     std::string rep = "{";
     auto pick = [&](const char* key, int which, int count, auto set, auto name) -> int {
-        // time every shape of one kernel in the sequence, round-robin; keep the best
+        // time every shape of one kernel in the sequence, in turn; keep the best
         std::vector<float> t(count, 1e30f);
-        for (int r = 0; r < rounds; ++r)
+        for (int r = 0; r < passes; ++r)
             for (int s = 0; s < count; ++s) {
                 CHK(set(s));
                 float ms = 0;
